@@ -1,0 +1,176 @@
+/*
+ * sampo_engine.h -- C ABI of the fused, batched agent-step engine (gfx950).
+ *
+ * One call of sampo_engine_step() advances every admitted scenario instance
+ * by reference iterations (one simulated hour each, src/abms.c:15-16); each
+ * iteration is the body of the reference loop src/abms.c:502-529:
+ * calcStats -> generateSeeds -> killAgents -> updateAgents -> initAgents ->
+ * 3x (preScan + segmented scan) -> oldToNewAgents.  The engine runs it as one
+ * fused pass per iteration over structure-of-arrays storage, for any number
+ * of instances at once, and keeps every observable result of the reference:
+ * the per-iteration printStats row (src/abms.c:67-81), the population ranges
+ * (struct Population, include/agent.h:109-123), every agent's fields in the
+ * reference's padded, state-sorted index order, and the age histogram
+ * (kernel/ageHistogram.cl).
+ *
+ * All functions return 0 on success or a negative SAMPO_E* code; the message
+ * of the last failure is available from sampo_last_error().
+ */
+#ifndef SAMPO_ENGINE_H
+#define SAMPO_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAMPO_E_ARG (-1)     /* invalid argument */
+#define SAMPO_E_HIP (-2)     /* HIP runtime error */
+#define SAMPO_E_NOMEM (-3)   /* device or host allocation failed */
+#define SAMPO_E_FULL (-4)    /* no free instance slot */
+#define SAMPO_E_STATE (-5)   /* call not valid in the current state */
+
+/* instance status */
+#define SAMPO_RUNNING 0u
+#define SAMPO_DONE 1u          /* horizon reached */
+#define SAMPO_CAPACITY 2u      /* nAgents > 0.95 * maxAgents (src/abms.c:305) */
+#define SAMPO_EXTINCT 3u       /* nAgents == 0 (src/abms.c:307) */
+#define SAMPO_PHYS_FULL 4u     /* physical slots of the instance exhausted */
+#define SAMPO_EMPTY 0xffu      /* slot not in use */
+
+#define SAMPO_MODE_REFERENCE 0u /* killAgents.cl formulas, initial eggs via initAgents */
+#define SAMPO_MODE_CONFIG 1u    /* constant per-stage daily mortality, explicit initial population */
+
+/* struct Environment (include/agent.h:22-35), same field order, 24 bytes */
+typedef struct {
+	float bloodmealSuccess;
+	float ITNValue;
+	float IRSValue;
+	float carryingCapacity;
+	float larvacideValue;
+	float oviTrapValue;
+} sampo_env;
+
+/* One agent in host form: the union of struct Agent, AgentAge and the state
+ * (include/agent.h:66-102).  `aux` holds the union ovipositionAttemps /
+ * cumulativeLarvalDelay as raw bits.  State is implied by the range. */
+typedef struct {
+	float ageInHours;
+	float hoursInState;
+	float delay;
+	float csd;           /* cumulativeSporogonicDevelopment */
+	uint32_t aux;
+	uint32_t availableEggs;
+	uint32_t cycleLength;
+	uint32_t humanBloodmealCount;
+	uint32_t numEggBatches;
+	uint32_t isFemale;
+} sampo_agent;
+
+/* Agent ranges of struct Population, in state order egg..gravid. */
+typedef struct {
+	uint32_t start[8];
+	uint32_t end[8];
+} sampo_layout;
+
+/* printStats row (src/abms.c:67-81) taken at the start of an iteration. */
+typedef struct {
+	uint32_t count[8];     /* eggs, larvae, pupae, immature, mating, BMS, BMD, gravid */
+	uint32_t l1de;         /* numLarvae1DazEquiv (0 without larvae) */
+	uint32_t females;
+	uint32_t potInfective;
+	uint32_t numCycles;    /* BitesNcycles of the previous iteration */
+	uint32_t sumCycles;
+	uint32_t bites;
+	uint32_t infectBites;
+	uint32_t valid;        /* 1 if the reference would have printed this row */
+} sampo_row;
+
+typedef struct {
+	uint32_t mode;          /* SAMPO_MODE_* */
+	uint32_t seed;          /* srand() seed of the host seed stream (src/abms.c:611) */
+	uint32_t horizon;       /* iterations to run */
+	uint32_t n_eggs;        /* reference mode: initial eggs (src/abms.c:18) */
+	uint64_t max_agents;    /* maxAgents of the 95% capacity check (include/abms.h:16) */
+	uint64_t phys_slots;    /* physical agent slots per buffer; 0 = derived from max_agents */
+	float dmr[4];           /* config mode: daily mortality egg, larva, pupa, adult */
+	const float *temp;      /* horizon temperatures, one per iteration */
+	const sampo_env *env;   /* horizon environments, or one if env_count == 1 */
+	uint32_t env_count;
+	/* config mode initial population, reference layout (start multiple-of-64
+	 * rule of kernel/oldToNewAgents.cl:140-160 or any layout with gaps >= 1);
+	 * agents dense in range order (sum of range sizes). */
+	const sampo_layout *init_layout;
+	const sampo_agent *init_agents;
+} sampo_scenario;
+
+typedef struct sampo_engine sampo_engine;
+
+const char *sampo_last_error(void);
+
+/* Create an engine on HIP device `device` with room for max_slots instances. */
+int sampo_engine_create(int device, uint32_t max_slots, sampo_engine **out);
+void sampo_engine_destroy(sampo_engine *e);
+
+/* Tile shape of the fused pass: work-group size (64..1024, multiple of 64)
+ * and agents per thread (1..8).  Only shapes compiled in are accepted. */
+int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt);
+
+/* Admit a scenario into a free slot: allocates its device state, uploads its
+ * inputs and prepares iteration 0.  Returns the slot in *slot. */
+int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uint32_t *slot);
+
+/* Free a slot (its instance must not be running in an unsynchronised step). */
+int sampo_engine_release(sampo_engine *e, uint32_t slot);
+
+/* Enqueue n iterations for all running instances; asynchronous. */
+int sampo_engine_step(sampo_engine *e, uint32_t n);
+
+/* Capture one day (24 iterations) as a HIP graph on first use and launch it
+ * `days` times; asynchronous. */
+int sampo_engine_step_days(sampo_engine *e, uint32_t days);
+
+int sampo_engine_sync(sampo_engine *e);
+
+/* Time n iterations with HIP events on the engine's stream (synchronises). */
+int sampo_engine_time(sampo_engine *e, uint32_t n, int use_graph, float *ms);
+
+/* Status, current iteration and agents alive at its start (synchronises). */
+int sampo_engine_status(sampo_engine *e, uint32_t slot, uint32_t *status, uint32_t *step,
+                        uint64_t *n_agents);
+
+/* Copy rows [first, first+n) of the instance's stats series (synchronises). */
+int sampo_engine_rows(sampo_engine *e, uint32_t slot, uint32_t first, uint32_t n, sampo_row *out);
+
+/* Materialise the instance's current agents in reference index order:
+ * layout receives the ranges; agents receives sum(end-start) records, dense
+ * in range order, newborn eggs included (synchronises). */
+int sampo_engine_export(sampo_engine *e, uint32_t slot, sampo_layout *layout, sampo_agent *agents,
+                        uint64_t max_agents);
+
+/* ageHistogram (kernel/ageHistogram.cl:13-35) of the current adults. */
+int sampo_engine_histogram(sampo_engine *e, uint32_t slot, uint32_t hist[100]);
+
+/* Sum over instances of agents alive at iteration starts since the last reset. */
+int sampo_engine_agent_steps(sampo_engine *e, uint64_t *total, int reset);
+
+/* Host-side reference seed stream: the first n glibc rand() values after
+ * srand(seed) (src/abms.c:140-158,611). */
+void sampo_seed_stream(uint32_t seed, uint32_t n, uint32_t *out);
+
+/* Synthetic configuration-mode inputs (a splitmix64 stream independent of the
+ * seed stream).  Daily temperature mean + amp*sin(2*pi*d/365) + noise*N(0,1),
+ * held for the 24 iterations of each day. */
+int sampo_gen_temperature(uint64_t seed, uint32_t days, float mean, float amp, float noise, float *hourly);
+
+/* n0 agents: egg/larva/pupa/immature with probability .3/.3/.1/.3, sex 50/50,
+ * part of each state's delay elapsed; laid out like the reference (ranges
+ * start at the next multiple of 64 after the previous end).  `agents` must
+ * hold n0 records; they come back dense in range order. */
+int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_layout *layout, sampo_agent *agents);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

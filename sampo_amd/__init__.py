@@ -1,0 +1,236 @@
+"""sampo_amd -- MI355X-native agent-step engine for SAMPO's mosquito model.
+
+Python host binding over the C ABI of libsampo_engine.so (include/
+sampo_engine.h).  The compute runs only in the HIP library: importing this
+package never falls back to a CPU implementation, and every call fails
+loudly when the library or a GPU is missing.
+
+The engine advances scenario instances by reference iterations (one
+simulated hour, src/abms.c:15-16); a day is 24 iterations.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ENGINE_LIB = os.path.join(HERE, "libsampo_engine.so")
+
+RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, EMPTY = 0, 1, 2, 3, 4, 0xFF
+MODE_REFERENCE, MODE_CONFIG = 0, 1
+STATES = ("egg", "larva", "pupa", "immature", "mateseeking", "bms", "bmd", "gravid")
+
+
+class SampoError(RuntimeError):
+    pass
+
+
+class Env(C.Structure):
+    """struct Environment (include/agent.h:22-35)."""
+    _fields_ = [("bloodmealSuccess", C.c_float), ("ITNValue", C.c_float), ("IRSValue", C.c_float),
+                ("carryingCapacity", C.c_float), ("larvacideValue", C.c_float),
+                ("oviTrapValue", C.c_float)]
+
+
+class Agent(C.Structure):
+    _fields_ = [("ageInHours", C.c_float), ("hoursInState", C.c_float), ("delay", C.c_float),
+                ("csd", C.c_float), ("aux", C.c_uint32), ("availableEggs", C.c_uint32),
+                ("cycleLength", C.c_uint32), ("humanBloodmealCount", C.c_uint32),
+                ("numEggBatches", C.c_uint32), ("isFemale", C.c_uint32)]
+
+
+class Layout(C.Structure):
+    _fields_ = [("start", C.c_uint32 * 8), ("end", C.c_uint32 * 8)]
+
+
+class Row(C.Structure):
+    _fields_ = [("count", C.c_uint32 * 8), ("l1de", C.c_uint32), ("females", C.c_uint32),
+                ("potInfective", C.c_uint32), ("numCycles", C.c_uint32), ("sumCycles", C.c_uint32),
+                ("bites", C.c_uint32), ("infectBites", C.c_uint32), ("valid", C.c_uint32)]
+
+
+class Scenario(C.Structure):
+    _fields_ = [("mode", C.c_uint32), ("seed", C.c_uint32), ("horizon", C.c_uint32),
+                ("n_eggs", C.c_uint32), ("max_agents", C.c_uint64), ("phys_slots", C.c_uint64),
+                ("dmr", C.c_float * 4), ("temp", C.POINTER(C.c_float)), ("env", C.POINTER(Env)),
+                ("env_count", C.c_uint32), ("init_layout", C.POINTER(Layout)),
+                ("init_agents", C.POINTER(Agent))]
+
+
+AGENT_DTYPE = np.dtype([("ageInHours", "<f4"), ("hoursInState", "<f4"), ("delay", "<f4"),
+                        ("csd", "<f4"), ("aux", "<u4"), ("availableEggs", "<u4"),
+                        ("cycleLength", "<u4"), ("humanBloodmealCount", "<u4"),
+                        ("numEggBatches", "<u4"), ("isFemale", "<u4")])
+ROW_DTYPE = np.dtype([("count", "<u4", (8,)), ("l1de", "<u4"), ("females", "<u4"),
+                      ("potInfective", "<u4"), ("numCycles", "<u4"), ("sumCycles", "<u4"),
+                      ("bites", "<u4"), ("infectBites", "<u4"), ("valid", "<u4")])
+assert AGENT_DTYPE.itemsize == C.sizeof(Agent) and ROW_DTYPE.itemsize == C.sizeof(Row)
+
+_lib = None
+
+
+def lib():
+    """Load libsampo_engine.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ENGINE_LIB):
+            raise SampoError(f"{ENGINE_LIB} is not built; run python -m sampo_amd.build")
+        L = C.CDLL(ENGINE_LIB)
+        vp = C.c_void_p
+        L.sampo_last_error.restype = C.c_char_p
+        L.sampo_engine_create.argtypes = [C.c_int, C.c_uint32, C.POINTER(vp)]
+        L.sampo_engine_destroy.argtypes = [vp]
+        L.sampo_engine_destroy.restype = None
+        L.sampo_engine_set_tile.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.sampo_engine_admit.argtypes = [vp, C.POINTER(Scenario), C.POINTER(C.c_uint32)]
+        L.sampo_engine_release.argtypes = [vp, C.c_uint32]
+        L.sampo_engine_step.argtypes = [vp, C.c_uint32]
+        L.sampo_engine_step_days.argtypes = [vp, C.c_uint32]
+        L.sampo_engine_sync.argtypes = [vp]
+        L.sampo_engine_time.argtypes = [vp, C.c_uint32, C.c_int, C.POINTER(C.c_float)]
+        L.sampo_engine_status.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                          C.POINTER(C.c_uint64)]
+        L.sampo_engine_rows.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp]
+        L.sampo_engine_export.argtypes = [vp, C.c_uint32, C.POINTER(Layout), vp, C.c_uint64]
+        L.sampo_engine_histogram.argtypes = [vp, C.c_uint32, vp]
+        L.sampo_engine_agent_steps.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
+        L.sampo_seed_stream.argtypes = [C.c_uint32, C.c_uint32, vp]
+        L.sampo_seed_stream.restype = None
+        L.sampo_gen_temperature.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float, vp]
+        L.sampo_gen_population.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.POINTER(Layout), vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SampoError(f"sampo error {rc}: {lib().sampo_last_error().decode()}")
+
+
+def seed_stream(seed, n):
+    """The first n glibc rand() values after srand(seed) (src/abms.c:140-158)."""
+    out = np.empty(n, dtype=np.uint32)
+    lib().sampo_seed_stream(seed, n, out.ctypes.data)
+    return out
+
+
+def gen_temperature(seed, days, mean=25.0, amp=5.0, noise=1.5):
+    out = np.empty(days * 24, dtype=np.float32)
+    _check(lib().sampo_gen_temperature(seed, days, mean, amp, noise, out.ctypes.data))
+    return out
+
+
+def gen_population(seed, n0, t0):
+    """Configuration-mode initial population: (layout (8,2) array, agents)."""
+    lay = Layout()
+    agents = np.zeros(n0, dtype=AGENT_DTYPE)
+    _check(lib().sampo_gen_population(seed, n0, t0, C.byref(lay), agents.ctypes.data))
+    return np.array([[lay.start[s], lay.end[s]] for s in range(8)], dtype=np.uint32), agents
+
+
+def make_env(bloodmeal=0.25, itn=0.0, irs=0.0, cc=160000.0, larvicide=0.0, ovitrap=0.0):
+    return np.array([(bloodmeal, itn, irs, cc, larvicide, ovitrap)],
+                    dtype=[(n, "<f4") for n, _ in Env._fields_])
+
+
+class Engine:
+    """A set of scenario instances advanced together on one GPU."""
+
+    def __init__(self, device=0, max_slots=16):
+        h = C.c_void_p()
+        _check(lib().sampo_engine_create(device, max_slots, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib().sampo_engine_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tile(self, wg, apt):
+        _check(lib().sampo_engine_set_tile(self.h, wg, apt))
+
+    def admit(self, *, mode=MODE_REFERENCE, seed=1, horizon, temp, env, n_eggs=0, max_agents,
+              phys_slots=0, dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):
+        temp = np.ascontiguousarray(temp, dtype=np.float32)
+        env = np.ascontiguousarray(env)
+        if temp.shape[0] < horizon:
+            raise ValueError("temp shorter than horizon")
+        sc = Scenario()
+        sc.mode, sc.seed, sc.horizon, sc.n_eggs = mode, seed, horizon, n_eggs
+        sc.max_agents, sc.phys_slots = max_agents, phys_slots
+        sc.dmr = (C.c_float * 4)(*dmr)
+        sc.temp = temp.ctypes.data_as(C.POINTER(C.c_float))
+        sc.env = C.cast(env.ctypes.data, C.POINTER(Env))
+        sc.env_count = env.shape[0]
+        lay = None
+        if layout is not None:
+            lay = Layout()
+            for s in range(8):
+                lay.start[s], lay.end[s] = int(layout[s][0]), int(layout[s][1])
+            agents = np.ascontiguousarray(agents, dtype=AGENT_DTYPE)
+            sc.init_layout = C.pointer(lay)
+            sc.init_agents = C.cast(agents.ctypes.data, C.POINTER(Agent))
+        slot = C.c_uint32()
+        _check(lib().sampo_engine_admit(self.h, C.byref(sc), C.byref(slot)))
+        return slot.value
+
+    def release(self, slot):
+        _check(lib().sampo_engine_release(self.h, slot))
+
+    def step(self, n):
+        _check(lib().sampo_engine_step(self.h, n))
+
+    def step_days(self, days):
+        _check(lib().sampo_engine_step_days(self.h, days))
+
+    def sync(self):
+        _check(lib().sampo_engine_sync(self.h))
+
+    def time(self, n, graph=False):
+        ms = C.c_float()
+        _check(lib().sampo_engine_time(self.h, n, 1 if graph else 0, C.byref(ms)))
+        return ms.value
+
+    def status(self, slot):
+        st, step, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().sampo_engine_status(self.h, slot, C.byref(st), C.byref(step), C.byref(n)))
+        return st.value, step.value, n.value
+
+    def rows(self, slot, first, n):
+        out = np.zeros(n, dtype=ROW_DTYPE)
+        _check(lib().sampo_engine_rows(self.h, slot, first, n, out.ctypes.data))
+        return out
+
+    def export(self, slot):
+        """(layout (8,2) uint32 array, dense agents in reference index order)."""
+        lay = Layout()
+        _check(lib().sampo_engine_export(self.h, slot, C.byref(lay), None, 0))
+        layout = np.array([[lay.start[s], lay.end[s]] for s in range(8)], dtype=np.uint32)
+        n = int((layout[:, 1] - layout[:, 0]).sum())
+        agents = np.zeros(n, dtype=AGENT_DTYPE)
+        if n:
+            _check(lib().sampo_engine_export(self.h, slot, C.byref(lay), agents.ctypes.data, n))
+        return layout, agents
+
+    def histogram(self, slot):
+        out = np.zeros(100, dtype=np.uint32)
+        _check(lib().sampo_engine_histogram(self.h, slot, out.ctypes.data))
+        return out
+
+    def agent_steps(self, reset=False):
+        v = C.c_uint64()
+        _check(lib().sampo_engine_agent_steps(self.h, C.byref(v), 1 if reset else 0))
+        return v.value

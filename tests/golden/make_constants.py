@@ -1,0 +1,71 @@
+"""Write tests/golden/constants.json: numeric model constants read as text
+from the reference sources (include/properties.h, kernel/killAgents.cl,
+src/abms.c).  The JSON is data -- numbers with the file:line they come from --
+used to pin the oracle's transcription of those constants.
+
+Run in the container that has the reference mounted:
+    python tests/golden/make_constants.py /root/reference
+"""
+import json
+import os
+import re
+import sys
+
+
+def grab(path, pattern, flags=0):
+    text = open(path).read()
+    m = re.search(pattern, text, flags)
+    if not m:
+        raise SystemExit(f"pattern not found in {path}: {pattern}")
+    line = text[:m.start()].count("\n") + 1
+    return m, line
+
+
+def main(ref):
+    props = os.path.join(ref, "include", "properties.h")
+    out = {}
+    m, line = grab(props, r"devRates\[\]\s*=\s*\{([^}]*)\}", re.S)
+    nums = [float(x) for x in re.findall(r"[0-9.eE+-]+", re.sub(r"//.*", "", m.group(1)))]
+    out["devRates"] = {"values": nums, "source": f"include/properties.h:{line}"}
+    m, line = grab(props, r"REAL eggsHatching\(REAL probability\) \{(.*?)return", re.S)
+    body = m.group(1)
+    thr = [float(x.rstrip("f")) for x in re.findall(r"probability < ([0-9.]+f?)", body)]
+    ms = [float(x.rstrip("f")) for x in re.findall(r"m\s*=\s*(-?[0-9.]+f?);", body)]
+    bs = [float(x.rstrip("f")) for x in re.findall(r"b\s*=\s*(-?[0-9.]+f?);", body)]
+    out["eggsHatching"] = {"thresholds": thr, "m": ms[1:], "b": bs[1:], "source": f"include/properties.h:{line}"}
+
+    def linear(name):
+        mm, ln = grab(props, name + r"\(Temperature temperature\) \{\s*return\s*([^;]*);", re.S)
+        return {"expr": mm.group(1).strip(), "source": f"include/properties.h:{ln}"}
+
+    for fn in ("eggsIncubation", "pupaDelay", "immatureDelay", "bloodMealDigestingDelay"):
+        out[fn] = linear("REAL " + fn)
+    m, line = grab(props, r"REAL eggBatchSize = ([0-9.]+)f;\s*REAL stdDev = ([0-9.]+)f;", re.S)
+    out["generateEggs"] = {"mean": float(m.group(1)), "sd": float(m.group(2)), "source": f"include/properties.h:{line}"}
+    m, line = grab(props, r"REAL mean = ([0-9.]+)f;\s*REAL stdDev = ([0-9.]+)f;", re.S)
+    out["larvaDelay"] = {"mean": float(m.group(1)), "sd": float(m.group(2)), "source": f"include/properties.h:{line}"}
+    kill = os.path.join(ref, "kernel", "killAgents.cl")
+    m, line = grab(kill, r"a = ([0-9.]+)f;\s*B = ([0-9.]+)f;\s*s = ([0-9.]+)f;", re.S)
+    out["adultMortality"] = {"a": float(m.group(1)), "B": float(m.group(2)), "s": float(m.group(3)),
+                             "source": f"kernel/killAgents.cl:{line}"}
+    gr = os.path.join(ref, "include", "gpuRand.h")
+    m, line = grab(gr, r"return ([0-9.e-]+)f \*")
+    out["rngScale"] = {"literal": m.group(1), "source": f"include/gpuRand.h:{line}"}
+    steps = re.findall(r"TausStep\(z(\d), (\d+), (\d+), (\d+), (\d+)UL\)", open(gr).read())
+    out["taus"] = [[int(a) for a in s] for s in steps]
+    lcg = re.search(r"LCGStep\(z4, (\d+), (\d+)UL\)", open(gr).read())
+    out["lcg"] = [int(lcg.group(1)), int(lcg.group(2))]
+    abms = os.path.join(ref, "src", "abms.c")
+    txt = open(abms).read()
+    out["abms"] = {"FACTOR": int(re.search(r"#define FACTOR (\d+)", txt).group(1)),
+                   "initialAgentCount": re.search(r"UINT initialAgentCount = ([^;]*);", txt).group(1),
+                   "carryingCapacity": re.search(r"REAL carryingCapacity = ([^;]*);", txt).group(1),
+                   "maxSteps": int(re.search(r"long long maxSteps = (\d+);", txt).group(1))}
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "constants.json")
+    with open(dst, "w") as f:
+        json.dump(out, f, indent=1)
+    print(dst)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
