@@ -136,6 +136,15 @@ int sampo_engine_sync(sampo_engine *e);
 /* Time n iterations with HIP events on the engine's stream (synchronises). */
 int sampo_engine_time(sampo_engine *e, uint32_t n, int use_graph, float *ms);
 
+/* Enqueue n iterations with HIP events around every fused pass; returns the
+ * wall time from the first to the last pass and the summed pass durations
+ * (synchronises). */
+int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_ms, float *pass_ms);
+
+/* Device-to-device copy of `bytes` bytes with a plain 16-byte-per-lane
+ * kernel, `iters` times; average milliseconds per copy. */
+int sampo_stream_copy(int device, uint64_t bytes, uint32_t iters, float *ms_per_copy);
+
 /* Status, current iteration and agents alive at its start (synchronises). */
 int sampo_engine_status(sampo_engine *e, uint32_t slot, uint32_t *status, uint32_t *step,
                         uint64_t *n_agents);

@@ -17,7 +17,12 @@
 
 #define LOCAL_SIZE 64u   /* include/agent.h:16 */
 #define SCAN_WX 128u     /* src/abms.c:497 SCAN_SIZE = 2 * LOCAL_SIZE */
-#define MARGIN 1024u     /* slack past capacity for the +1 scan slots */
+#define MARGIN 1024u     /* slack past the storage for the +1 scan slots */
+/* Storage beyond maxAgents: the reference writes new eggs and survivors with
+ * no bound check (src/abms.c:385-386) and only asserts capacity at the next
+ * calcStats; the restatement keeps STORAGE x capacity slots so that it stops
+ * at that same check, not at its own allocation. */
+#define STORAGE 4u
 
 enum { R_EGG, R_LARVA, R_PUPA, R_IMM, R_MS, R_BMS, R_BMD, R_GRAVID };
 
@@ -524,7 +529,7 @@ static int k_old_to_new(or_sim *s)
 	o->r[R_BMD].end = o->r[R_BMD].start + (uint32_t)(p1[pop.r[R_BMD].end] + 1);
 	o->r[R_GRAVID].start = next_start(o->r[R_BMD].end);
 	o->r[R_GRAVID].end = o->r[R_GRAVID].start + (uint32_t)(p2[pop.r[R_GRAVID].end] + 1);
-	if (o->r[R_GRAVID].end + 2u > s->cfg.capacity + MARGIN) return 1;
+	if ((uint64_t)o->r[R_GRAVID].end + 2u > (uint64_t)STORAGE * s->cfg.capacity + MARGIN) return 1;
 
 	const or_agent *oa = s->agents[s->cur];
 	const or_age *og = s->ages[s->cur];
@@ -582,7 +587,7 @@ static int alloc_sim(or_sim *s, const or_config *cfg)
 {
 	memset(s, 0, sizeof(*s));
 	s->cfg = *cfg;
-	size_t n = (size_t)cfg->capacity + MARGIN;
+	size_t n = (size_t)STORAGE * cfg->capacity + MARGIN;
 	for (int b = 0; b < 2; ++b) {
 		s->agents[b] = (or_agent *)calloc(n, sizeof(or_agent));
 		s->ages[b] = (or_age *)calloc(n, sizeof(or_age));
@@ -718,7 +723,7 @@ int or_sim_run(or_sim *s, uint32_t n, or_stats *rows)
 		k_kill_agents(s, global, env->carryingCapacity);
 		k_update_agents(s, global, env, t, h);
 		/* createNewAgents(): src/abms.c:381-462 */
-		if (s->enb.newEggs > s->cfg.capacity) { s->status = 1; break; }
+		if ((uint64_t)s->enb.newEggs > (uint64_t)STORAGE * s->cfg.capacity) { s->status = 1; break; }
 		k_init_agents(s, s->cur ^ 1, s->enb.newEggs, t);
 		const or_pop *ph = &s->pop[0];
 		if (s->cfg.scan_path == 0) {

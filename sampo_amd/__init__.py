@@ -94,6 +94,8 @@ def lib():
         L.sampo_engine_export.argtypes = [vp, C.c_uint32, C.POINTER(Layout), vp, C.c_uint64]
         L.sampo_engine_histogram.argtypes = [vp, C.c_uint32, vp]
         L.sampo_engine_agent_steps.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
+        L.sampo_engine_time_pass.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.sampo_stream_copy.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]
         L.sampo_seed_stream.argtypes = [C.c_uint32, C.c_uint32, vp]
         L.sampo_seed_stream.restype = None
         L.sampo_gen_temperature.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float, vp]
@@ -126,6 +128,13 @@ def gen_population(seed, n0, t0):
     agents = np.zeros(n0, dtype=AGENT_DTYPE)
     _check(lib().sampo_gen_population(seed, n0, t0, C.byref(lay), agents.ctypes.data))
     return np.array([[lay.start[s], lay.end[s]] for s in range(8)], dtype=np.uint32), agents
+
+
+def stream_copy_ms(nbytes, iters=20, device=0):
+    """Average ms of a plain device-to-device copy of nbytes (read + write)."""
+    ms = C.c_float()
+    _check(lib().sampo_stream_copy(device, nbytes, iters, C.byref(ms)))
+    return ms.value
 
 
 def make_env(bloodmeal=0.25, itn=0.0, irs=0.0, cc=160000.0, larvicide=0.0, ovitrap=0.0):
@@ -204,6 +213,12 @@ class Engine:
         _check(lib().sampo_engine_time(self.h, n, 1 if graph else 0, C.byref(ms)))
         return ms.value
 
+    def time_pass(self, n):
+        """(wall ms, summed fused-pass ms) over n iterations."""
+        w, p = C.c_float(), C.c_float()
+        _check(lib().sampo_engine_time_pass(self.h, n, C.byref(w), C.byref(p)))
+        return w.value, p.value
+
     def status(self, slot):
         st, step, n = C.c_uint32(), C.c_uint32(), C.c_uint64()
         _check(lib().sampo_engine_status(self.h, slot, C.byref(st), C.byref(step), C.byref(n)))
@@ -234,3 +249,71 @@ class Engine:
         v = C.c_uint64()
         _check(lib().sampo_engine_agent_steps(self.h, C.byref(v), 1 if reset else 0))
         return v.value
+
+
+# ---------------------------------------------------------------- icl path
+
+ICL_LIB = os.path.join(HERE, "libsampo_icl.so")
+
+
+class AbmsParams(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("n_eggs", C.c_uint32), ("max_agents", C.c_uint32), ("steps", C.c_uint32),
+                ("serial_scan", C.c_uint32), ("temp", C.POINTER(C.c_float)), ("env", C.POINTER(Env)),
+                ("env_count", C.c_uint32)]
+
+
+class AbmsResult(C.Structure):
+    _fields_ = [("rows", C.c_void_p), ("status", C.c_uint32), ("steps_done", C.c_uint32),
+                ("agent_steps", C.c_uint64), ("layout", Layout), ("histogram", C.c_uint32 * 100),
+                ("loop_ms", C.c_double), ("total_ms", C.c_double), ("agents", C.c_void_p), ("ages", C.c_void_p),
+                ("states", C.c_void_p), ("span_capacity", C.c_uint64)]
+
+
+ICL_AGENT = np.dtype([("availableEggs", "<u4"), ("cycleLength", "<u4"), ("humanBloodmealCount", "<u4"),
+                      ("delay", "<f4"), ("numEggBatches", "<u4"), ("aux", "<u4")])
+ICL_AGE = np.dtype([("ageInHours", "<f4"), ("hoursInState", "<f4"), ("isFemale", "<u4"), ("csd", "<f4")])
+ICL_STATE = np.dtype([("dead", "<u4"), ("state", "<u4")])
+
+_icl = None
+
+
+def icl_lib():
+    """Load libsampo_icl.so (the icl_* boundary and the reference driver loop)."""
+    global _icl
+    if _icl is None:
+        if not os.path.exists(ICL_LIB):
+            raise SampoError(f"{ICL_LIB} is not built; run python -m sampo_amd.build")
+        L = C.CDLL(ICL_LIB)
+        L.sampo_abms_run.argtypes = [C.POINTER(AbmsParams), C.POINTER(AbmsResult)]
+        _icl = L
+    return _icl
+
+
+def run_abms(*, seed, n_eggs, max_agents, steps, temp, env, serial_scan=False, export=True):
+    """Run the reference driver loop on the GPU through the icl boundary.
+
+    Returns a dict with rows, status, steps_done, agent_steps, layout,
+    histogram, loop_ms and (export=True) the final AoS agent arrays."""
+    temp = np.ascontiguousarray(temp, dtype=np.float32)
+    env = np.ascontiguousarray(env)
+    p = AbmsParams(seed, n_eggs, max_agents, steps, 1 if serial_scan else 0,
+                   temp.ctypes.data_as(C.POINTER(C.c_float)), C.cast(env.ctypes.data, C.POINTER(Env)), env.shape[0])
+    rows = np.zeros(steps, dtype=ROW_DTYPE)
+    r = AbmsResult()
+    r.rows = rows.ctypes.data
+    if export:
+        ag = np.zeros(max_agents, dtype=ICL_AGENT)
+        ge = np.zeros(max_agents, dtype=ICL_AGE)
+        st = np.zeros(max_agents, dtype=ICL_STATE)
+        r.agents, r.ages, r.states, r.span_capacity = ag.ctypes.data, ge.ctypes.data, st.ctypes.data, max_agents
+    rc = icl_lib().sampo_abms_run(C.byref(p), C.byref(r))
+    if rc != 0:
+        raise SampoError(f"sampo_abms_run failed with {rc}")
+    layout = np.array([[r.layout.start[s], r.layout.end[s]] for s in range(8)], dtype=np.uint32)
+    out = dict(rows=rows[:r.steps_done], status=r.status, steps_done=r.steps_done, agent_steps=r.agent_steps,
+               layout=layout, histogram=np.array(r.histogram[:], dtype=np.uint32), loop_ms=r.loop_ms,
+               total_ms=r.total_ms)
+    if export:
+        span = int(layout[7][1])
+        out.update(agents=ag[:span], ages=ge[:span], states=st[:span])
+    return out

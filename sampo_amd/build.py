@@ -1,6 +1,9 @@
 """Compile the gfx950 libraries of the package in-tree.
 
 libsampo_engine.so  fused, batched agent-step engine (csrc/sampo_engine.hip)
+libsampo_icl.so     the reference's icl_* runtime API, scan API and kernel
+                    contract over HIP, plus the reference driver loop
+                    (csrc/sampo_icl.hip, csrc/sampo_abms.c)
 
 Every HIP source is compiled with -ffp-contract=off and IEEE division/sqrt so
 that device arithmetic follows the canonical numeric contract bit for bit,
@@ -24,14 +27,32 @@ HIP_FLAGS = [
 
 TARGETS = {
     "libsampo_engine.so": ["sampo_engine.hip"],
+    "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c"],
 }
+C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 
 def build(verbose=False):
     out = []
     for lib, srcs in TARGETS.items():
         dst = os.path.join(HERE, lib)
-        cmd = ["hipcc", *HIP_FLAGS, "-o", dst, *[os.path.join(CSRC, s) for s in srcs]]
+        objs = []
+        for s in srcs:
+            if s.endswith(".c"):  # host C: gcc, canonical flags
+                obj = os.path.join(HERE, "_build", s[:-2] + ".o")
+                os.makedirs(os.path.dirname(obj), exist_ok=True)
+                subprocess.run(["gcc", *C_FLAGS, "-c", "-o", obj, os.path.join(CSRC, s)], check=True)
+                objs.append(obj)
+        hip_srcs = [os.path.join(CSRC, s) for s in srcs if not s.endswith(".c")]
+        cmd = ["hipcc", *HIP_FLAGS, "-o", dst, *hip_srcs]
+        if objs:  # hipcc reads every input as HIP source: link host objects in a second step
+            hobj = os.path.join(HERE, "_build", lib + ".hip.o")
+            cmd = ["hipcc", *[f for f in HIP_FLAGS if f != "-shared"], "-c", "-o", hobj, *hip_srcs]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise RuntimeError(f"hipcc failed for {lib}")
+            cmd = ["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", dst, hobj, *objs]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)

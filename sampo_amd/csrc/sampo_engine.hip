@@ -978,11 +978,11 @@ static int enqueue(sampo_engine *e, uint32_t n)
 {
 	const uint32_t tile = e->wg * e->apt;
 	for (uint32_t i = 0; i < n; ++i) {
-		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(128), 0, e->stream, e->d_inst);
 		hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
 		                   e->d_pl_par, e->max_slots, tile);
 		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots, &e->d_lb};
 		HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
+		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(128), 0, e->stream, e->d_inst);
 	}
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1254,5 +1254,76 @@ extern "C" int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_
 		layout->end[s] = start + n;
 		end = start + n;
 	}
+	return 0;
+}
+
+/* ============================================================ measurement */
+
+extern "C" int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_ms, float *pass_ms)
+{
+	if (!e || !wall_ms || !pass_ms || n == 0) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	const uint32_t tile = e->wg * e->apt;
+	std::vector<hipEvent_t> ev(2 * (size_t)n);
+	for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+	for (uint32_t i = 0; i < n; ++i) {
+		hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
+		                   e->d_pl_par, e->max_slots, tile);
+		HIPCHK(hipEventRecord(ev[2 * i], e->stream));
+		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots, &e->d_lb};
+		HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
+		HIPCHK(hipEventRecord(ev[2 * i + 1], e->stream));
+		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(128), 0, e->stream, e->d_inst);
+	}
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipEventSynchronize(ev[2 * (size_t)n - 1]));
+	float sum = 0.0f;
+	for (uint32_t i = 0; i < n; ++i) {
+		float ms = 0.0f;
+		HIPCHK(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+		sum += ms;
+	}
+	HIPCHK(hipEventElapsedTime(wall_ms, ev[0], ev[2 * (size_t)n - 1]));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	for (auto &x : ev) (void)hipEventDestroy(x);
+	*pass_ms = sum;
+	return 0;
+}
+
+/* plain device-to-device copy, 16 bytes per lane per step: the achievable
+ * HBM ceiling and the reference point for traffic counters */
+__global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+	for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dst[i] = src[i];
+}
+
+extern "C" int sampo_stream_copy(int device, uint64_t bytes, uint32_t iters, float *ms_per_copy)
+{
+	if (!ms_per_copy || bytes < 16 || iters == 0) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
+	HIPCHK(hipSetDevice(device));
+	size_t n = (size_t)(bytes / 16);
+	uint4 *a = nullptr, *b = nullptr;
+	HIPCHK(hipMalloc(&a, n * 16));
+	HIPCHK(hipMalloc(&b, n * 16));
+	HIPCHK(hipMemset(a, 1, n * 16));
+	int cus = 0;
+	HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+	dim3 grid((unsigned)(cus * 16));
+	hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, 0, a, b, n);
+	hipEvent_t x, y;
+	HIPCHK(hipEventCreate(&x));
+	HIPCHK(hipEventCreate(&y));
+	HIPCHK(hipEventRecord(x, 0));
+	for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, 0, a, b, n);
+	HIPCHK(hipEventRecord(y, 0));
+	HIPCHK(hipEventSynchronize(y));
+	float ms = 0.0f;
+	HIPCHK(hipEventElapsedTime(&ms, x, y));
+	*ms_per_copy = ms / (float)iters;
+	(void)hipEventDestroy(x);
+	(void)hipEventDestroy(y);
+	(void)hipFree(a);
+	(void)hipFree(b);
 	return 0;
 }

@@ -1,0 +1,183 @@
+"""GPU parity: the HIP paths against the CPU oracle on the same inputs.
+
+Bar: bit-exact.  Every printStats row, the final ranges, every field of
+every agent in reference index order, and the age histogram must equal the
+oracle's.  Sizes are chosen so the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import sampo_amd as S
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROWF = ("l1de", "females", "potInfective", "numCycles", "sumCycles", "bites", "infectBites")
+
+
+def rows_equal(orows, grows):
+    assert len(orows) == len(grows)
+    assert np.array_equal(orows["count"], grows["count"])
+    for f in ROWF:
+        assert np.array_equal(orows[f], grows[f]), f
+
+
+def agents_equal(a, b):
+    assert a.shape == b.shape
+    for f in a.dtype.names:
+        assert np.array_equal(a[f], b[f]), f
+
+
+def ref_inputs(steps, cc, temp=30.0):
+    return np.full(steps, temp, np.float32), S.make_env(cc=cc)
+
+
+@pytest.mark.parametrize("n_eggs,cc,days,seed", [(3200, 16000.0, 12, 1), (500, 3000.0, 20, 7), (64, 800.0, 6, 3)])
+def test_engine_reference_mode_matches_oracle(n_eggs, cc, days, seed):
+    steps = 24 * days
+    temp, env = ref_inputs(steps, cc)
+    sim = O.Sim(temp=temp, env=env, capacity=400_000, seed=seed, n_eggs=n_eggs)
+    orows = sim.run(steps)
+    with S.Engine(max_slots=2) as e:
+        slot = e.admit(mode=S.MODE_REFERENCE, seed=seed, horizon=steps, temp=temp, env=env, n_eggs=n_eggs,
+                       max_agents=400_000)
+        e.step(steps)
+        st, step, n = e.status(slot)
+        assert st == S.DONE and step == steps
+        rows_equal(orows, e.rows(slot, 0, steps))
+        lay_o, ag_o, _ = sim.export()
+        lay_g, ag_g = e.export(slot)
+        assert np.array_equal(lay_o, lay_g)
+        agents_equal(ag_o, ag_g)
+        assert np.array_equal(sim.histogram(), e.histogram(slot))
+
+
+def config_inputs(n0, days, seed, mean=25.0, amp=5.0):
+    temp = S.gen_temperature(seed, days, mean=mean, amp=amp)
+    lay, ag = S.gen_population(seed, n0, float(temp[0]))
+    return temp, lay, ag
+
+
+@pytest.mark.parametrize("n0,days,seed,dmr", [(20000, 10, 11, (0.05, 0.03, 0.02, 0.1)),
+                                               (3000, 15, 12, (0.04, 0.045, 0.01, 0.12))])
+def test_engine_config_mode_matches_oracle(n0, days, seed, dmr):
+    steps = 24 * days
+    temp, lay, ag = config_inputs(n0, days, seed)
+    env = S.make_env(cc=5.0 * n0)
+    sim = O.Sim(temp=temp, env=env, capacity=60 * n0, seed=seed, layout=lay, agents=ag, mode=1, dmr=dmr)
+    orows = sim.run(steps)
+    with S.Engine(max_slots=2) as e:
+        slot = e.admit(mode=S.MODE_CONFIG, seed=seed, horizon=steps, temp=temp, env=env, max_agents=60 * n0,
+                       dmr=dmr, layout=lay, agents=ag)
+        e.step(steps)
+        rows_equal(orows, e.rows(slot, 0, steps))
+        lay_o, ag_o, _ = sim.export()
+        lay_g, ag_g = e.export(slot)
+        assert np.array_equal(lay_o, lay_g)
+        agents_equal(ag_o, ag_g)
+
+
+def test_engine_hip_graph_day_equals_plain_steps():
+    steps = 24 * 6
+    temp, env = ref_inputs(steps, 8000.0)
+    out = []
+    for graph in (False, True):
+        with S.Engine(max_slots=1) as e:
+            slot = e.admit(seed=5, horizon=steps, temp=temp, env=env, n_eggs=1600, max_agents=1_000_000)
+            if graph:
+                e.step_days(steps // 24)
+            else:
+                e.step(steps)
+            out.append((e.rows(slot, 0, steps + 1), e.export(slot)[1]))
+    rows_equal(out[0][0], out[1][0])
+    agents_equal(out[0][1], out[1][1])
+
+
+def test_engine_batched_instances_equal_solo_runs():
+    """Instances of different sizes, seeds and modes in one launch per
+    iteration are each bit-identical to running alone."""
+    steps = 24 * 5
+    specs = []
+    for i, (n, cc) in enumerate([(3000, 15000.0), (200, 1000.0), (9000, 40000.0), (40, 300.0)]):
+        temp, env = ref_inputs(steps, cc, temp=26.0 + i)
+        specs.append(dict(mode=S.MODE_REFERENCE, seed=100 + i, horizon=steps, temp=temp, env=env, n_eggs=n,
+                          max_agents=50 * n + 10000))
+    temp, lay, ag = config_inputs(5000, 5, 21)
+    specs.append(dict(mode=S.MODE_CONFIG, seed=21, horizon=steps, temp=temp, env=S.make_env(cc=25000.0),
+                      max_agents=300000, dmr=(0.05, 0.03, 0.02, 0.1), layout=lay, agents=ag))
+    solo = []
+    for sp in specs:
+        with S.Engine(max_slots=1) as e:
+            s = e.admit(**sp)
+            e.step(steps)
+            solo.append((e.rows(s, 0, steps + 1), e.export(s)[1]))
+    with S.Engine(max_slots=8) as e:
+        slots = [e.admit(**sp) for sp in specs]
+        e.step_days(steps // 24)
+        for s, (r, a) in zip(slots, solo):
+            rows_equal(r, e.rows(s, 0, steps + 1))
+            agents_equal(a, e.export(s)[1])
+
+
+def test_engine_tile_shapes_agree():
+    steps = 24 * 4
+    temp, env = ref_inputs(steps, 20000.0)
+    ref = None
+    for wg, apt in ((256, 1), (128, 2), (64, 4), (512, 2), (256, 4)):
+        with S.Engine(max_slots=1) as e:
+            e.set_tile(wg, apt)
+            s = e.admit(seed=9, horizon=steps, temp=temp, env=env, n_eggs=5000, max_agents=1_000_000)
+            e.step(steps)
+            got = (e.rows(s, 0, steps + 1), e.export(s)[1])
+        if ref is None:
+            ref = got
+        else:
+            rows_equal(ref[0], got[0])
+            agents_equal(ref[1], got[1])
+
+
+def test_engine_status_capacity_and_extinction():
+    steps = 24 * 30
+    temp, env = ref_inputs(steps, 20000.0)
+    with S.Engine(max_slots=2) as e:
+        s = e.admit(seed=1, horizon=steps, temp=temp, env=env, n_eggs=2000, max_agents=2200)
+        e.step(steps)
+        st, step, _ = e.status(s)
+        assert st == S.CAPACITY and step < steps
+        t2, lay, ag = config_inputs(400, 2, 5)
+        s2 = e.admit(mode=S.MODE_CONFIG, seed=5, horizon=48, temp=t2, env=S.make_env(cc=2000.0),
+                     max_agents=10000, dmr=(1.0, 1.0, 1.0, 1.0), layout=lay, agents=ag)
+        e.step(48)
+        st2, step2, _ = e.status(s2)
+        assert st2 == S.EXTINCT and step2 < 48
+    sim = O.Sim(temp=temp, env=env, capacity=2200, seed=1, n_eggs=2000)
+    sim.run(steps)
+    assert sim.status == 1 and sim.step == step
+
+
+def icl_vs_oracle(n_eggs, cc, steps, seed, serial):
+    temp, env = ref_inputs(steps, cc)
+    cap = 400_000
+    sim = O.Sim(temp=temp, env=env, capacity=cap, seed=seed, n_eggs=n_eggs)
+    orows = sim.run(steps)
+    g = S.run_abms(seed=seed, n_eggs=n_eggs, max_agents=cap, steps=steps, temp=temp, env=env, serial_scan=serial)
+    assert g["status"] == 0 and g["steps_done"] == steps
+    rows_equal(orows, g["rows"])
+    lay_o = sim.layout()
+    assert np.array_equal(lay_o, g["layout"])
+    _, ag_o, st_o = sim.export()
+    idx = np.concatenate([np.arange(lay_o[r][0], lay_o[r][1]) for r in range(8)])
+    for f in ("availableEggs", "cycleLength", "humanBloodmealCount", "delay", "numEggBatches", "aux"):
+        assert np.array_equal(g["agents"][f][idx], ag_o[f]), f
+    for f in ("ageInHours", "hoursInState", "isFemale", "csd"):
+        assert np.array_equal(g["ages"][f][idx], ag_o[f]), f
+    assert np.array_equal(g["states"]["state"][idx], st_o)
+    assert np.array_equal(g["histogram"], sim.histogram())
+
+
+def test_icl_driver_parallel_scan_matches_oracle():
+    icl_vs_oracle(3200, 16000.0, 24 * 10, 1, serial=False)
+
+
+def test_icl_driver_serial_scan_matches_oracle():
+    icl_vs_oracle(300, 2000.0, 24 * 6, 4, serial=True)
