@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30, help="timed days")
     ap.add_argument("--warmup", type=int, default=2, help="untimed days before the timed region")
     ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
-    ap.add_argument("--tile", default="256x1", help="work-group size x agents per thread")
+    ap.add_argument("--tile", default="512x1x6", help="work-group size x agents per thread x min waves per SIMD")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-days", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=10)
@@ -111,9 +111,9 @@ def main():
     if args.warmup + args.steps > w["days"]:
         sys.exit("warmup + steps exceeds the workload's horizon")
     temp, env, layout, agents = build_inputs(w)
-    wg, apt = (int(x) for x in args.tile.split("x"))
+    wg, apt, waves = (int(x) for x in (args.tile.split("x") + ["0"])[:3])
     with S.Engine(max_slots=1) as e:
-        e.set_tile(wg, apt)
+        e.set_tile(wg, apt, waves)
         slot = e.admit(mode=S.MODE_CONFIG, seed=w["seed"], horizon=24 * w["days"], temp=temp, env=env,
                        max_agents=w["max_agents"], dmr=DMR, layout=layout, agents=agents)
         del agents

@@ -113,9 +113,16 @@ const char *sampo_last_error(void);
 int sampo_engine_create(int device, uint32_t max_slots, sampo_engine **out);
 void sampo_engine_destroy(sampo_engine *e);
 
-/* Tile shape of the fused pass: work-group size (64..1024, multiple of 64)
- * and agents per thread (1..8).  Only shapes compiled in are accepted. */
-int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt);
+/* Tile shape of the fused pass: work-group size, agents per thread and the
+ * minimum waves per SIMD the registers are capped for (0: no cap).  Only
+ * shapes compiled into the library are accepted (SAMPO_E_ARG otherwise). */
+int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt, uint32_t waves);
+
+/* Chunking of instances admitted from now on.  A chunk is the run of agents
+ * one work-group takes and the granularity of output segments: at least
+ * `agents` agents (default 4096), grown so that an instance has at most
+ * `target_chunks` chunks (0: four per resident work-group of the pass). */
+int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chunks);
 
 /* Admit a scenario into a free slot: allocates its device state, uploads its
  * inputs and prepares iteration 0.  Returns the slot in *slot. */

@@ -81,7 +81,8 @@ def lib():
         L.sampo_engine_create.argtypes = [C.c_int, C.c_uint32, C.POINTER(vp)]
         L.sampo_engine_destroy.argtypes = [vp]
         L.sampo_engine_destroy.restype = None
-        L.sampo_engine_set_tile.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.sampo_engine_set_tile.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.sampo_engine_set_chunk.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.sampo_engine_admit.argtypes = [vp, C.POINTER(Scenario), C.POINTER(C.c_uint32)]
         L.sampo_engine_release.argtypes = [vp, C.c_uint32]
         L.sampo_engine_step.argtypes = [vp, C.c_uint32]
@@ -168,8 +169,15 @@ class Engine:
         except Exception:
             pass
 
-    def set_tile(self, wg, apt):
-        _check(lib().sampo_engine_set_tile(self.h, wg, apt))
+    def set_tile(self, wg, apt, waves=0):
+        """Work-group size, agents per thread, register cap (min waves/SIMD)."""
+        _check(lib().sampo_engine_set_tile(self.h, wg, apt, waves))
+
+    def set_chunk(self, agents, target_chunks=0):
+        """Minimum agents per chunk (work unit / output segment) and the most
+        chunks per instance (0: four per resident work-group) for later
+        admissions."""
+        _check(lib().sampo_engine_set_chunk(self.h, agents, target_chunks))
 
     def admit(self, *, mode=MODE_REFERENCE, seed=1, horizon, temp, env, n_eggs=0, max_agents,
               phys_slots=0, dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):

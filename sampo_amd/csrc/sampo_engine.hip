@@ -1,29 +1,34 @@
 /*
  * sampo_engine.hip -- fused, batched per-iteration agent step for gfx950.
  *
- * One iteration of the reference loop (src/abms.c:502-529) is two launches:
+ * One iteration of the reference loop (src/abms.c:502-529) is three launches
+ * on one stream:
  *
- *   k_plan  (1 work-group)  lays every running instance's tiles end to end
- *                           and resets the tile ticket;
- *   k_pass  (persistent)    work-groups take tiles in ticket order; each tile
- *                           reads its agents once, applies killAgents and
- *                           updateAgents (kernel/killAgents.cl:24-148,
- *                           kernel/updateAgents.cl:39-270), ranks the
- *                           survivors by new state with 64-lane ballots and
- *                           an LDS scan, gets the counts of all earlier tiles
- *                           of its instance by decoupled look-back, and writes
- *                           each survivor once to its final slot -- the
- *                           compaction of preScan + scan + oldToNewAgents
- *                           (kernel/preScan.cl, kernel/boltScan.cl,
- *                           kernel/oldToNewAgents.cl) without materialising
- *                           any prefix array.
+ *   k_plan  (1 work-group)   lays every running instance's chunks end to end
+ *                            and resets the chunk ticket;
+ *   k_pass  (persistent)     work-groups take chunks by ticket.  A chunk is a
+ *                            contiguous run of the instance's agents in
+ *                            reference index order; each agent is read once,
+ *                            killed and updated (kernel/killAgents.cl:24-148,
+ *                            kernel/updateAgents.cl:39-270) in registers,
+ *                            ranked by its new state with 64-lane ballots and
+ *                            an LDS scan, and written once into the chunk's
+ *                            output segment for that state.  No work-group
+ *                            waits on another;
+ *   k_prep  (1 group/instance) folds the chunks' counts and sums into the new
+ *                            segment list -- the compaction of preScan + scan
+ *                            + oldToNewAgents (kernel/preScan.cl,
+ *                            kernel/boltScan.cl, kernel/oldToNewAgents.cl) --
+ *                            computes the next ranges, printStats row and
+ *                            calcStats sums (kernel/calcL1de.cl,
+ *                            kernel/calcGender.cl), the larval mortality
+ *                            table, and cuts the next chunks.
  *
- * The look-back also carries the sums that calcStats (kernel/calcL1de.cl,
- * kernel/calcGender.cl) and the bites/cycles/newEggs atomics need, so the
- * instance's last tile holds the totals after its own look-back and prepares
- * the next iteration in place (layout, printStats row, mortality table).
- * New eggs are not written (see sampo_model.h); the next pass regenerates
- * them from the seeds that laid them, which is what initAgents computes.
+ * Survivors keep reference order: chunks are in old-index order, and an
+ * output segment (chunk c, state s) holds c's survivors of state s in
+ * old-index order; the new state-s range is those segments for c = 0, 1, ...
+ * which is the stable rank the reference's scans compute.  New eggs are not
+ * written (see sampo_model.h).
  */
 #include <hip/hip_runtime.h>
 
@@ -36,44 +41,67 @@
 
 /* ------------------------------------------------------------ device state */
 
+#define PB_NEWBORN 0xffffffffu
+
+struct Seg { /* a run of one state, stored contiguously (or the newborns) */
+	uint32_t k0;     /* dense index of its first agent */
+	uint32_t state;
+	uint32_t pbase;  /* physical slot of its first agent, PB_NEWBORN for newborns */
+	uint32_t vbase;  /* virtual index of its first agent */
+};
+
+struct Chunk {
+	uint32_t k0, k1; /* dense range */
+	uint32_t seg0;   /* segment holding k0 */
+	uint32_t pad;
+	uint32_t pout[8]; /* output base per new state, in the other buffer */
+};
+
+/* per-chunk results, structure of arrays: cout[q * cmax + c] for
+ * q < 8 survivors per new state, then newEggs, bites, infectious bites,
+ * cycles, cycle sum, L1DE, females, potentially infective */
+#define NOUT 16
+
+struct IterState {
+	uint32_t step, active, cur, n;
+	uint32_t cnt[8], vstart[8], kstart[9];
+	uint32_t nseg, nchunk, enew, nb_set;
+	uint32_t cs;           /* agents per chunk of this iteration */
+	uint32_t seg0_done;    /* chunk seg0 already filled by the fold */
+	float nb_temp;
+	uint32_t tb, h, seed_set;
+	float temp;
+	sampo_env env;
+	float hmr_egg, hmr_pupa;
+	uint32_t pad;
+	double csd_inc;        /* 1/((111/(T-16))*14) of updateAgents.cl:69-70, in binary64 */
+	float hmr_larva[100];
+};
+
 struct Inst {
 	uint32_t *buf[2];      /* SoA: field f of buffer b at buf[b] + f * P */
-	uint32_t P;
-	uint32_t horizon;
-	uint32_t mode;
-	uint32_t env_count;
-	uint64_t max_agents;
+	Seg *seg[2];           /* iteration with parity q reads seg[q] ... */
+	Chunk *chunk[2];       /* ... and chunk[q] */
+	uint32_t *cout;
 	const uint32_t *seeds; /* (horizon + 1) sets of 16 values */
 	const float *temp;
 	const sampo_env *env;
 	sampo_row *rows;       /* horizon + 1 */
+	uint64_t max_agents;
+	uint32_t P, horizon, mode, env_count;
+	uint32_t chunk_size, target_chunks, cmax, smax;
+	uint32_t status, step, pending, needs_begin;
+	uint32_t l1de, females, potinf, ncyc, scyc, bites, ibites; /* stats of the start of `step` */
 	float hmr_stage[4];
 	float hmr_adult[100];
-	uint32_t status;
-	uint32_t step;
-	uint32_t pending;      /* tot holds the totals of a finished iteration */
-	uint32_t tot[16];      /* 8 new-state counts, newEggs, bites, infectious bites,
-	                          cycles, cycle sum, L1DE, females, potentially infective */
-	sampo_iter p[2];       /* iteration step uses p[step & 1] */
+	IterState st[2];       /* iteration `step` uses st[step & 1] */
 };
 
 struct Ctx {
 	uint32_t ticket;
 	uint32_t total;
-	uint32_t epoch;
-	uint32_t pad;
 	unsigned long long agent_steps;
 };
-
-#define LB_AGG 1ull
-#define LB_INC 2ull
-#define NV 16 /* look-back values per tile: 8 state counts + 8 sums */
-
-__device__ static inline void lb_store(uint64_t *p, uint64_t kind, uint32_t epoch, uint32_t v)
-{
-	uint64_t w = (kind << 62) | ((uint64_t)(epoch & 0x3fffffffu) << 32) | v;
-	__hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 /* ------------------------------------------------------------ the agent */
 
@@ -95,15 +123,17 @@ struct Acc {
 	uint32_t eggs, bites, ibites, ncyc, scyc;
 };
 
-/* per-tile copy of the instance's iteration parameters */
+/* per-chunk copy of the instance's iteration parameters */
 struct TileParams {
-	sampo_iter ip;
+	IterState ip;
 	float hmr_adult[100];
 	uint32_t seed[16];
 	uint32_t nbseed[8];   /* seeds[2], seeds[3] of the set that laid the newborns */
 	uint32_t *bin, *bout;
-	uint32_t P;
-	uint32_t mode;
+	const Seg *seg;
+	uint32_t *cout;
+	Chunk ch;
+	uint32_t P, nseg, cmax;
 };
 
 __device__ static inline float rnd(const uint32_t *z, uint32_t g)
@@ -130,7 +160,7 @@ __device__ __attribute__((noinline)) uint32_t gen_eggs(float u0, float u1, uint3
  * Returns the new state, or S_DEAD. */
 __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, const TileParams &T, Acc &acc)
 {
-	const sampo_iter &ip = T.ip;
+	const IterState &ip = T.ip;
 	/* killAgents.cl:55-147 */
 	uint32_t d = sc_f2u(sc_minf(a.age / 24.0f, 99.0f));
 	float hmr = s == S_EGG ? ip.hmr_egg : s == S_LARVA ? ip.hmr_larva[d] : s == S_PUPA ? ip.hmr_pupa
@@ -142,11 +172,9 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	a.age += 1.0f;
 	a.his += 1.0f;
 	uint32_t hbc = a.misc & 0xffffu;
-	{
+	{ /* the increment depends on T only; k_prep evaluates it once per iteration */
 		int cond = (t > 16.0f && hbc > 0u);
-		float q = 111.0f / (t - 16.0f);
-		double x = 1.0 / ((double)q * 14.0);
-		a.csd = (float)((double)a.csd + (double)cond * x);
+		a.csd = (float)((double)a.csd + (double)cond * ip.csd_inc);
 	}
 	const uint32_t h = ip.h;
 	switch (s) {
@@ -255,66 +283,6 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	}
 }
 
-/* ------------------------------------------------------------ k_plan */
-
-__global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par,
-                                               uint32_t ninst, uint32_t tile)
-{
-	__shared__ uint32_t s_scan[1024];
-	__shared__ unsigned long long s_steps[1024];
-	const uint32_t tid = threadIdx.x;
-	uint32_t carry = 0;
-	unsigned long long steps = 0;
-	for (uint32_t b = 0; b < ninst; b += 1024u) {
-		uint32_t i = b + tid;
-		uint32_t nt = 0;
-		if (i < ninst) {
-			const Inst &I = inst[i];
-			uint32_t par = I.step & 1u;
-			const sampo_iter &ip = I.p[par];
-			if (I.status == SAMPO_RUNNING && ip.active) {
-				nt = (ip.n + tile - 1u) / tile;
-				steps += ip.n;
-			}
-			pl_par[i] = (uint8_t)par;
-		}
-		s_scan[tid] = nt;
-		__syncthreads();
-		for (uint32_t off = 1; off < 1024u; off <<= 1) {
-			uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
-			__syncthreads();
-			s_scan[tid] += v;
-			__syncthreads();
-		}
-		if (i < ninst) pl_base[i] = carry + s_scan[tid] - nt;
-		carry += s_scan[1023];
-		__syncthreads();
-	}
-	s_steps[tid] = steps;
-	__syncthreads();
-	for (uint32_t off = 512; off > 0; off >>= 1) {
-		if (tid < off) s_steps[tid] += s_steps[tid + off];
-		__syncthreads();
-	}
-	if (tid == 0) {
-		pl_base[ninst] = carry;
-		ctx->total = carry;
-		ctx->ticket = 0u;
-		ctx->epoch = ctx->epoch + 1u;
-		ctx->agent_steps += s_steps[0];
-	}
-}
-
-/* ------------------------------------------------------------ k_pass */
-
-__device__ static inline uint32_t range_of(const uint32_t *kstart, uint32_t k)
-{
-	uint32_t s = 0;
-#pragma unroll
-	for (int r = 1; r < 8; ++r) s += (k >= kstart[r]) ? 1u : 0u;
-	return s;
-}
-
 __device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uint32_t p)
 {
 	a.age = sc_float(b[p]);
@@ -341,7 +309,7 @@ __device__ static inline void store_agent(const Ag &a, uint32_t *b, size_t P, ui
 
 /* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3) at
  * temperature t */
-__device__ __host__ static inline void newborn(Ag &a, const uint32_t *z, float t, uint32_t g)
+__device__ static inline void newborn(Ag &a, const uint32_t *z, float t, uint32_t g)
 {
 	a.age = 0.0f;
 	a.his = 0.0f;
@@ -362,83 +330,437 @@ __device__ static inline uint32_t wave_sum(uint32_t v)
 	return v;
 }
 
-/* k_prep: one work-group per instance whose iteration just finished.  From
- * the totals the last tile left it builds the layout that oldToNewAgents.cl
- * :121-160 writes to popD[1] and runs calcStats (src/abms.c:230-310) of the
- * next iteration: printStats row, capacity and extinction checks, biomass,
- * and the larval mortality table of killAgents.cl:117-133. */
-__global__ void __launch_bounds__(128) k_prep(Inst *insts)
+/* last segment j in [lo, hi) with seg k0 <= k (segments sorted by k0) */
+__device__ static inline uint32_t seg_find(const uint32_t *k0s, uint32_t lo, uint32_t hi, uint32_t k)
 {
-	__shared__ uint32_t s_go, s_l1de;
-	__shared__ float s_cc;
-	Inst *I = insts + blockIdx.x;
-	if (threadIdx.x == 0) {
-		s_go = 0;
-		if (I->status == SAMPO_RUNNING && I->pending) {
-			const uint32_t par = I->step & 1u;
-			const sampo_iter &ip = I->p[par];
-			const uint32_t *tot = I->tot;
-			sampo_prep_in in;
-			in.step = ip.step + 1u;
-			in.cur = ip.cur ^ 1u;
-			uint32_t e = tot[8];
-			for (int s = 0; s < 8; ++s) {
-				in.cnt[s] = tot[s] + (s == 0 ? e : 0u);
-				in.phys_in[s] = ip.pout[s];
-			}
-			uint32_t vend = in.cnt[0];
-			in.vstart[0] = 0;
-			for (int s = 1; s < 8; ++s) {
-				in.vstart[s] = ((vend + SAMPO_PAD) / SAMPO_PAD) * SAMPO_PAD;
-				vend = in.vstart[s] + in.cnt[s];
-			}
-			in.enew = e;
-			in.bites = tot[9];
-			in.ibites = tot[10];
-			in.ncyc = tot[11];
-			in.scyc = tot[12];
-			in.l1de = tot[13];
-			in.females = tot[14];
-			in.potinf = tot[15];
-			uint32_t tn = in.step < I->horizon ? in.step : I->horizon - 1u;
-			in.temp_t = I->temp[tn];
-			in.temp_prev = ip.temp;
-			in.env_t = I->env[I->env_count == 1u ? 0u : tn];
-			sampo_iter *o = &I->p[par ^ 1u];
-			uint32_t st = sm_prep(I->mode, I->horizon, I->max_agents, I->P, I->hmr_stage, &in, o,
-			                      &I->rows[in.step]);
-			I->status = st;
-			I->step = in.step;
-			I->pending = 0u;
-			s_go = st == SAMPO_RUNNING ? 1u + (I->mode == SAMPO_MODE_REFERENCE ? 1u : 0u) : 0u;
-			s_l1de = in.l1de;
-			s_cc = in.env_t.carryingCapacity;
-		}
+	while (hi - lo > 1u) {
+		uint32_t mid = (lo + hi) >> 1;
+		if (k0s[mid] <= k) lo = mid; else hi = mid;
 	}
+	return lo;
+}
+
+/* ------------------------------------------------------------ k_plan */
+
+__global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par,
+                                               uint32_t ninst)
+{
+	__shared__ uint32_t s_scan[1024];
+	__shared__ unsigned long long s_steps[1024];
+	const uint32_t tid = threadIdx.x;
+	uint32_t carry = 0;
+	unsigned long long steps = 0;
+	for (uint32_t b = 0; b < ninst; b += 1024u) {
+		uint32_t i = b + tid;
+		uint32_t nc = 0;
+		if (i < ninst) {
+			const Inst &I = inst[i];
+			uint32_t par = I.step & 1u;
+			const IterState &ip = I.st[par];
+			if (I.status == SAMPO_RUNNING && ip.active) {
+				nc = ip.nchunk;
+				steps += ip.n;
+			}
+			pl_par[i] = (uint8_t)par;
+		}
+		s_scan[tid] = nc;
+		__syncthreads();
+		for (uint32_t off = 1; off < 1024u; off <<= 1) {
+			uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
+			__syncthreads();
+			s_scan[tid] += v;
+			__syncthreads();
+		}
+		if (i < ninst) pl_base[i] = carry + s_scan[tid] - nc;
+		carry += s_scan[1023];
+		__syncthreads();
+	}
+	s_steps[tid] = steps;
 	__syncthreads();
-	if (s_go) {
-		sampo_iter *o = &I->p[I->step & 1u];
-		for (uint32_t d = threadIdx.x; d < 100u; d += blockDim.x)
-			o->hmr_larva[d] = s_go == 2u ? sm_hmr_larva(d, s_l1de, s_cc) : I->hmr_stage[1];
+	for (uint32_t off = 512; off > 0; off >>= 1) {
+		if (tid < off) s_steps[tid] += s_steps[tid + off];
+		__syncthreads();
+	}
+	if (tid == 0) {
+		pl_base[ninst] = carry;
+		ctx->total = carry;
+		ctx->ticket = 0u;
+		ctx->agent_steps += s_steps[0];
 	}
 }
 
-template <int WG, int APT>
-__global__ void __launch_bounds__(WG) k_pass(Inst *insts, Ctx *ctx, const uint32_t *pl_base,
-                                             const uint8_t *pl_par, uint32_t ninst, uint64_t *lb)
+/* ------------------------------------------------------------ k_prep */
+
+#define PREP_WG 1024
+
+/* exclusive scan of v over the work-group; returns the total in *total */
+__device__ static inline uint32_t block_excl(uint32_t v, uint32_t *lds, uint32_t *total)
+{
+	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+	uint32_t x = v;
+	for (int o = 1; o < 64; o <<= 1) {
+		uint32_t y = __shfl_up(x, o, 64);
+		if (lane >= (uint32_t)o) x += y;
+	}
+	if (lane == 63u) lds[w] = x;
+	__syncthreads();
+	uint32_t before = 0, all = 0;
+	for (uint32_t q = 0; q < PREP_WG / 64; ++q) {
+		if (q < w) before += lds[q];
+		all += lds[q];
+	}
+	__syncthreads();
+	*total = all;
+	return before + x - v;
+}
+
+
+/* chunk size of an iteration with N agents: at least chunk_size, and at
+ * most target_chunks chunks */
+__device__ static inline uint32_t chunk_agents(const Inst *I, uint32_t N)
+{
+	uint32_t cs = I->chunk_size;
+	if (I->target_chunks) {
+		uint32_t want = (uint32_t)(((uint64_t)N + I->target_chunks - 1u) / I->target_chunks);
+		want = sm_roundup(want, 256u);
+		if (want > cs) cs = want;
+	}
+	return cs;
+}
+
+/* Stage A: fold the finished iteration of I (parity par) into the segment
+ * list, ranges and sums of the next one (parity par ^ 1). */
+__device__ static void fold_iteration(Inst *I, uint32_t par)
+{
+	__shared__ uint32_t lds[PREP_WG / 64];
+	__shared__ uint32_t s_tot[NOUT], s_kst[9], s_vst[8], s_cs, s_C;
+	const uint32_t tid = threadIdx.x, lane = tid & 63u;
+	const IterState &ip = I->st[par];
+	IterState &np = I->st[par ^ 1u];
+	const uint32_t C = ip.nchunk, cmax = I->cmax;
+	const uint32_t *co = I->cout;
+	const Chunk *ch = I->chunk[par];
+	Chunk *nch = I->chunk[par ^ 1u];
+	Seg *seg = I->seg[par ^ 1u];
+	if (tid < NOUT) s_tot[tid] = 0u;
+	__syncthreads();
+	/* totals: each thread sums chunks tid, tid + PREP_WG, ... of every
+	 * counter row; loads are issued in batches so each batch waits once */
+	{
+		uint32_t v[NOUT];
+		for (int q = 0; q < NOUT; ++q) v[q] = 0;
+		for (uint32_t c = tid; c < C; c += PREP_WG) {
+			uint32_t x[NOUT];
+#pragma unroll
+			for (int q = 0; q < NOUT; ++q) x[q] = __builtin_nontemporal_load(co + (size_t)q * cmax + c);
+#pragma unroll
+			for (int q = 0; q < NOUT; ++q) v[q] += x[q];
+		}
+#pragma unroll
+		for (int q = 0; q < NOUT; ++q) {
+			uint32_t r = wave_sum(v[q]);
+			if (lane == 0 && r) atomicAdd(&s_tot[q], r);
+		}
+	}
+	__syncthreads();
+	const uint32_t E = s_tot[8];
+	if (tid == 0) {
+		uint32_t k = 0, vend = 0;
+		for (int s = 0; s < 8; ++s) {
+			uint32_t c = s_tot[s] + (s == 0 ? E : 0u);
+			np.cnt[s] = c;
+			s_kst[s] = k;
+			k += c;
+			uint32_t vs = s == 0 ? 0u : ((vend + SAMPO_PAD) / SAMPO_PAD) * SAMPO_PAD;
+			np.vstart[s] = vs;
+			s_vst[s] = vs;
+			vend = vs + c;
+		}
+		s_kst[8] = k;
+		np.enew = E;
+		np.cur = ip.cur ^ 1u;
+		np.step = ip.step + 1u;
+		I->bites = s_tot[9];
+		I->ibites = s_tot[10];
+		I->ncyc = s_tot[11];
+		I->scyc = s_tot[12];
+		I->l1de = s_tot[13];
+		I->females = s_tot[14];
+		I->potinf = s_tot[15];
+		const uint32_t cs = chunk_agents(I, k);
+		uint32_t nc = (uint32_t)(((uint64_t)k + cs - 1u) / cs);
+		s_cs = cs;
+		s_C = nc < cmax ? nc : cmax;
+		np.cs = cs;
+		np.seg0_done = 1u;
+	}
+	__syncthreads();
+	const uint32_t cs = s_cs, NC = s_C;
+	/* segments: newborns, then (state, chunk) in state-major order; each
+	 * thread takes a contiguous run of (state, chunk) entries.  Each new
+	 * chunk's first segment is recorded as its start is passed. */
+	const uint32_t head = E > 0u ? 1u : 0u;
+	if (head) {
+		if (tid == 0) {
+			Seg sg = {0u, 0u, PB_NEWBORN, 0u};
+			seg[0] = sg;
+		}
+		for (uint32_t c = tid; c < NC && c * cs < E; c += PREP_WG) nch[c].seg0 = 0u;
+	}
+	const uint32_t total = 8u * C;
+	const uint32_t L = (total + PREP_WG - 1u) / PREP_WG;
+	const uint32_t q0 = tid * L < total ? tid * L : total;
+	const uint32_t q1 = q0 + L < total ? q0 + L : total;
+	constexpr uint32_t B = 8; /* loads in flight per batch */
+	uint32_t sk = 0, si = 0;
+	for (uint32_t qb = q0; qb < q1; qb += B) {
+		uint32_t n[B];
+#pragma unroll
+		for (uint32_t u = 0; u < B; ++u) {
+			uint32_t q = qb + u;
+			uint32_t s = q / C, c = q - s * C;
+			n[u] = q < q1 ? co[(size_t)s * cmax + c] : 0u;
+		}
+#pragma unroll
+		for (uint32_t u = 0; u < B; ++u) {
+			sk += n[u];
+			si += n[u] ? 1u : 0u;
+		}
+	}
+	uint32_t tk, ti;
+	uint32_t k0 = E + block_excl(sk, lds, &tk);
+	uint32_t idx = head + block_excl(si, lds, &ti);
+	for (uint32_t qb = q0; qb < q1; qb += B) {
+		uint32_t n[B], pb[B];
+#pragma unroll
+		for (uint32_t u = 0; u < B; ++u) {
+			uint32_t q = qb + u < q1 ? qb + u : q0;
+			uint32_t s = q / C, c = q - s * C;
+			n[u] = qb + u < q1 ? co[(size_t)s * cmax + c] : 0u;
+			pb[u] = ch[c].pout[s];
+		}
+#pragma unroll
+		for (uint32_t u = 0; u < B; ++u) {
+			if (!n[u]) continue;
+			uint32_t q = qb + u;
+			uint32_t s = q / C;
+			Seg sg;
+			sg.k0 = k0;
+			sg.state = s;
+			sg.pbase = pb[u];
+			sg.vbase = s_vst[s] + (k0 - s_kst[s]);
+			seg[idx] = sg;
+			for (uint32_t nc = (k0 + cs - 1u) / cs; nc < NC && nc * cs < k0 + n[u]; ++nc) nch[nc].seg0 = idx;
+			++idx;
+			k0 += n[u];
+		}
+	}
+	if (tid == 0) np.nseg = head + ti;
+}
+
+/* Stage B: calcStats of the iteration in st[par] (src/abms.c:230-310), its
+ * printStats row, checks, per-iteration parameters and chunks. */
+__device__ static void begin_iteration(Inst *I, uint32_t par)
+{
+	__shared__ uint32_t lds[PREP_WG / 64];
+	__shared__ uint32_t s_status, s_go;
+	const uint32_t tid = threadIdx.x;
+	IterState &ip = I->st[par];
+	if (tid == 0) {
+		uint32_t n = 0;
+		for (int s = 0; s < 8; ++s) {
+			ip.kstart[s] = n;
+			n += ip.cnt[s];
+		}
+		ip.kstart[8] = n;
+		ip.n = n;
+		sampo_row row;
+		for (int s = 0; s < 8; ++s) row.count[s] = ip.cnt[s];
+		uint32_t adults_range = (ip.vstart[7] + ip.cnt[7]) - ip.vstart[3];
+		row.l1de = ip.cnt[1] > 0 ? I->l1de : 0u;
+		row.females = adults_range > 0 ? I->females : 0u;
+		row.potInfective = adults_range > 0 ? I->potinf : 0u;
+		row.numCycles = I->ncyc;
+		row.sumCycles = I->scyc;
+		row.bites = I->bites;
+		row.infectBites = I->ibites;
+		row.valid = 1u;
+		uint32_t st;
+		if (ip.step >= I->horizon) {
+			st = SAMPO_DONE;
+		} else if (!((double)n <= (double)I->max_agents * 0.95)) {
+			st = SAMPO_CAPACITY;
+			row.valid = 0u;
+		} else if (n == 0u) {
+			st = SAMPO_EXTINCT;
+			row.valid = 0u;
+		} else if (n > I->P) {
+			st = SAMPO_PHYS_FULL; /* before cutting chunks: at most cmax of them exist */
+		} else {
+			st = SAMPO_RUNNING;
+		}
+		I->rows[ip.step] = row;
+		const uint32_t t = ip.step < I->horizon ? ip.step : I->horizon - 1u;
+		const uint32_t tp = ip.step == 0u ? 0u : (ip.step - 1u < I->horizon ? ip.step - 1u : I->horizon - 1u);
+		ip.temp = I->temp[t];
+		{
+			float q = 111.0f / (ip.temp - 16.0f);
+			ip.csd_inc = 1.0 / ((double)q * 14.0);
+		}
+		ip.env = I->env[I->env_count == 1u ? 0u : t];
+		ip.nb_temp = I->temp[tp];
+		ip.nb_set = ip.step;
+		ip.seed_set = ip.step + 1u;
+		ip.h = sc_f2u((float)ip.step * 1.0f) % 24u;
+		ip.tb = I->l1de + (ip.vstart[0] + ip.cnt[0]) + ip.cnt[2]; /* killAgents.cl:42 */
+		if (I->mode == SAMPO_MODE_CONFIG) {
+			ip.hmr_egg = I->hmr_stage[0];
+			ip.hmr_pupa = I->hmr_stage[2];
+		} else {
+			ip.hmr_egg = sm_hmr(0.1f);
+			ip.hmr_pupa = sm_hmr(0.1f);
+		}
+		s_status = st;
+	}
+	__syncthreads();
+	uint32_t st = s_status;
+	/* chunks: equal runs of chunk_size agents; outputs bounded by the inputs
+	 * that can reach each state in one iteration */
+	const uint32_t N = ip.n;
+	const uint32_t cs = chunk_agents(I, N);
+	const uint32_t C = st == SAMPO_RUNNING ? (N + cs - 1u) / cs : 0u;
+	const bool have_seg0 = ip.seg0_done && ip.cs == cs;
+	Chunk *ch = I->chunk[par];
+	const Seg *seg = I->seg[par];
+	const uint32_t nseg = ip.nseg;
+	uint64_t carry = 0;
+	for (uint32_t b = 0; b < C; b += PREP_WG) {
+		uint32_t c = b + tid;
+		uint32_t size = 0;
+		uint32_t bound[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+		Chunk x;
+		if (c < C) {
+			x.k0 = c * cs;
+			x.k1 = x.k0 + cs < N ? x.k0 + cs : N;
+			uint32_t in[8];
+			for (int s = 0; s < 8; ++s) {
+				uint32_t a = ip.kstart[s] > x.k0 ? ip.kstart[s] : x.k0;
+				uint32_t z = ip.kstart[s + 1] < x.k1 ? ip.kstart[s + 1] : x.k1;
+				in[s] = z > a ? z - a : 0u;
+			}
+			bound[0] = in[0];
+			bound[1] = in[1] + in[0];
+			bound[2] = in[2] + in[1];
+			bound[3] = in[3] + in[2];
+			bound[4] = in[4] + in[3];
+			bound[5] = in[5] + in[4] + in[7];
+			bound[6] = in[6] + in[5];
+			bound[7] = in[7] + in[6];
+			for (int s = 0; s < 8; ++s) size += sm_roundup(bound[s], SAMPO_PAD);
+		}
+		uint32_t tot;
+		uint32_t base = block_excl(size, lds, &tot);
+		if (c < C) {
+			uint64_t p = carry + base;
+			for (int s = 0; s < 8; ++s) {
+				x.pout[s] = (uint32_t)p;
+				p += sm_roundup(bound[s], SAMPO_PAD);
+			}
+			/* seg0 was filled by the fold (or is filled below) */
+			ch[c].k0 = x.k0;
+			ch[c].k1 = x.k1;
+			for (int s = 0; s < 8; ++s) ch[c].pout[s] = x.pout[s];
+		}
+		carry += tot;
+	}
+	__syncthreads();
+	/* initial populations: segment holding each chunk's first agent, segment
+	 * j holding the chunks whose start falls in [k0_j, k0_(j+1)) */
+	if (!have_seg0)
+		for (uint32_t j = tid; j < nseg && C; j += PREP_WG) {
+			uint32_t a = seg[j].k0;
+			uint32_t z = j + 1u < nseg ? seg[j + 1u].k0 : N;
+			for (uint32_t c = (a + cs - 1u) / cs; c < C && c * cs < z; ++c) ch[c].seg0 = j;
+		}
+	if (tid == 0) {
+		if (st == SAMPO_RUNNING && carry > (uint64_t)I->P) st = SAMPO_PHYS_FULL;
+		ip.active = st == SAMPO_RUNNING;
+		ip.nchunk = ip.active ? C : 0u;
+		I->status = st;
+		I->step = ip.step;
+		s_go = st == SAMPO_RUNNING ? 1u + (I->mode == SAMPO_MODE_REFERENCE ? 1u : 0u) : 0u;
+	}
+	__syncthreads();
+	if (s_go)
+		for (uint32_t d = tid; d < 100u; d += PREP_WG)
+			ip.hmr_larva[d] = s_go == 2u ? sm_hmr_larva(d, I->l1de, ip.env.carryingCapacity) : I->hmr_stage[1];
+}
+
+/* The instance record is worked on in LDS: the bookkeeping is a chain of
+ * dependent scalar updates that would otherwise each wait on global memory. */
+__global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts)
+{
+	__shared__ Inst sI;
+	__shared__ uint32_t s_fold, s_begin, s_par;
+	Inst *G = insts + blockIdx.x;
+	{
+		const uint32_t *src = (const uint32_t *)G;
+		uint32_t *dst = (uint32_t *)&sI;
+		for (uint32_t q = threadIdx.x; q < sizeof(Inst) / 4u; q += PREP_WG) dst[q] = src[q];
+	}
+	__syncthreads();
+	Inst *I = &sI;
+	if (threadIdx.x == 0) {
+		s_fold = I->status == SAMPO_RUNNING && I->pending;
+		s_par = I->step & 1u;
+		s_begin = I->needs_begin;
+	}
+	__syncthreads();
+	if (!s_fold && !s_begin) return;
+	if (s_fold) {
+		fold_iteration(I, s_par);
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			I->pending = 0u;
+			I->step = I->step + 1u;
+			s_begin = 1u;
+			s_par ^= 1u;
+		}
+	}
+	__syncthreads();
+	if (s_begin) {
+		begin_iteration(I, s_par);
+		if (threadIdx.x == 0) I->needs_begin = 0u;
+	}
+	__syncthreads();
+	{
+		const uint32_t *src = (const uint32_t *)&sI;
+		uint32_t *dst = (uint32_t *)G;
+		for (uint32_t q = threadIdx.x; q < sizeof(Inst) / 4u; q += PREP_WG) dst[q] = src[q];
+	}
+}
+
+/* ------------------------------------------------------------ k_pass */
+
+template <int WG, int APT, int WAVES>
+__global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const uint32_t *pl_base,
+                                             const uint8_t *pl_par, uint32_t ninst)
 {
 	constexpr int NW = WG / 64;
 	constexpr uint32_t TILE = (uint32_t)WG * APT;
+	constexpr uint32_t SEGC = TILE + 64u; /* staged segment window: a tile overlaps <= TILE + 1 */
 	__shared__ TileParams T;
 	__shared__ uint32_t s_ticket, s_inst;
+	__shared__ uint32_t s_k0[SEGC], s_st[SEGC], s_pb[SEGC], s_vb[SEGC];
+	__shared__ uint32_t s_wlo, s_wm, s_wend; /* window: global index, size, dense end */
 	__shared__ uint32_t s_wc[APT][NW][8];
+	__shared__ uint32_t s_run[8], s_tc[8];
 	__shared__ uint32_t s_acc[NW][8];
-	__shared__ uint32_t s_agg[NV];
-	__shared__ uint32_t s_excl[NV];
+	__shared__ uint32_t s_out[NFIELD][TILE]; /* updated agents of the tile */
 
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-	const uint32_t epoch = ctx->epoch & 0x3fffffffu;
 	const uint32_t total = ctx->total;
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 
 	for (;;) {
 		if (tid == 0) {
@@ -457,150 +779,197 @@ __global__ void __launch_bounds__(WG) k_pass(Inst *insts, Ctx *ctx, const uint32
 		const uint32_t ticket = s_ticket;
 		if (ticket >= total) break;
 		const uint32_t ii = s_inst;
-		Inst *I = insts + ii;
-		const uint32_t par = pl_par[ii];
+		const uint32_t c = ticket - pl_base[ii];
 		{ /* parameters into LDS */
-			const uint32_t *src = (const uint32_t *)&I->p[par];
+			Inst *I = insts + ii;
+			const uint32_t par = pl_par[ii];
+			const uint32_t *src = (const uint32_t *)&I->st[par];
 			uint32_t *dst = (uint32_t *)&T.ip;
-			for (uint32_t q = tid; q < sizeof(sampo_iter) / 4u; q += WG) dst[q] = src[q];
+			for (uint32_t q = tid; q < sizeof(IterState) / 4u; q += WG) dst[q] = src[q];
 			for (uint32_t q = tid; q < 100u; q += WG) T.hmr_adult[q] = I->hmr_adult[q];
-			if (tid < 16u) T.seed[tid] = I->seeds[(size_t)(I->p[par].seed_set) * 16u + tid];
+			const uint32_t *csrc = (const uint32_t *)&I->chunk[par][c];
+			uint32_t *cdst = (uint32_t *)&T.ch;
+			for (uint32_t q = tid; q < sizeof(Chunk) / 4u; q += WG) cdst[q] = csrc[q];
+			if (tid < 16u) T.seed[tid] = I->seeds[(size_t)(I->st[par].seed_set) * 16u + tid];
 			if (tid >= 32u && tid < 40u)
-				T.nbseed[tid - 32u] = I->seeds[(size_t)(I->p[par].nb_set) * 16u + 8u + (tid - 32u)];
-			if (tid == 64u % WG) {
-				uint32_t cur = I->p[par].cur;
+				T.nbseed[tid - 32u] = I->seeds[(size_t)(I->st[par].nb_set) * 16u + 8u + (tid - 32u)];
+			if (tid == 0) {
+				uint32_t cur = I->st[par].cur;
 				T.bin = I->buf[cur];
 				T.bout = I->buf[cur ^ 1u];
+				T.seg = I->seg[par];
+				T.cout = I->cout;
+				T.cmax = I->cmax;
 				T.P = I->P;
-				T.mode = I->mode;
+				T.nseg = I->st[par].nseg;
+				s_wend = 0u; /* force staging at the first tile */
+				s_wlo = 0u;
+				s_wm = 0u;
 			}
+			if (tid < 8u) s_run[tid] = 0u;
 		}
 		__syncthreads();
-		const uint32_t first = pl_base[ii];
-		const uint32_t ntile_i = pl_base[ii + 1] - first; /* pl_base[ninst] = total */
-		const uint32_t tl = ticket - first;
-		const uint32_t N = T.ip.n;
 		const size_t P = T.P;
-
-		Ag a[APT];
-		uint32_t ns[APT], lr[APT];
 		Acc acc = {0, 0, 0, 0, 0};
 		uint32_t l1de = 0, fem = 0, pinf = 0;
-#pragma unroll
-		for (int j = 0; j < APT; ++j) {
-			const uint32_t k = tl * TILE + (uint32_t)j * WG + tid;
-			ns[j] = S_DEAD;
-			if (k < N) {
-				const uint32_t s = range_of(T.ip.kstart, k);
-				const uint32_t g = k + T.ip.voff[s];
-				if (s == S_EGG && k < T.ip.enew) newborn(a[j], T.nbseed, T.ip.nb_temp, g);
-				else load_agent(a[j], T.bin, P, k + T.ip.off_in[s]);
-				uint32_t n2 = agent_step(a[j], s, g, T, acc);
-				ns[j] = n2;
-				if (n2 == S_LARVA) l1de += sc_f2u(a[j].age / 24.0f);
-				if (n2 >= S_IMM && n2 < S_DEAD && (a[j].misc >> 31)) {
-					fem += 1u;
-					pinf += a[j].csd >= 1.0f ? 1u : 0u;
+		uint32_t next = T.ch.seg0; /* global index of the segment holding ts */
+
+		for (uint32_t ts = T.ch.k0; ts < T.ch.k1; ts += TILE) {
+			const uint32_t te = ts + TILE < T.ch.k1 ? ts + TILE : T.ch.k1;
+			if (te > s_wend && w == 0) { /* stage segments from the one holding ts */
+				uint32_t j = next, m = 0, kend = T.ip.n;
+				for (;;) {
+					uint32_t jj = j + lane;
+					Seg sg = {0u, 0u, 0u, 0u};
+					bool valid = jj < T.nseg;
+					if (valid) sg = T.seg[jj];
+					bool in = valid && sg.k0 < te;
+					uint64_t bl = __ballot(in);
+					uint32_t cnt = (uint32_t)__popcll(bl);
+					if (in) {
+						s_k0[m + lane] = sg.k0;
+						s_st[m + lane] = sg.state;
+						s_pb[m + lane] = sg.pbase;
+						s_vb[m + lane] = sg.vbase;
+					}
+					m += cnt;
+					if (cnt < 64u) { /* first lane past te holds the window end */
+						uint64_t past = __ballot(valid && !in);
+						if (past) kend = __shfl(sg.k0, (int)(__ffsll((unsigned long long)past) - 1), 64);
+						break;
+					}
+					if (m + 64u > SEGC) { /* window full: it ends where the next segment starts */
+						uint32_t nk = j + 64u < T.nseg ? T.seg[j + 64u].k0 : T.ip.n;
+						kend = nk;
+						break;
+					}
+					j += 64u;
+				}
+				if (lane == 0) {
+					s_wlo = next;
+					s_wm = m;
+					s_wend = kend;
 				}
 			}
-		}
-		/* rank survivors by new state: ballot per state, wave counts to LDS */
-		const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+			__syncthreads();
+			const uint32_t m = s_wm, wlo = s_wlo;
+			uint32_t ns[APT], lr[APT];
+			/* gather: all of the tile's loads issued before any update */
 #pragma unroll
-		for (int j = 0; j < APT; ++j) {
-			lr[j] = 0;
-#pragma unroll
-			for (uint32_t s = 0; s < 8u; ++s) {
-				uint64_t m = __ballot(ns[j] == s);
-				if (ns[j] == s) lr[j] = (uint32_t)__popcll(m & lt);
-				if (lane == 0) s_wc[j][w][s] = (uint32_t)__popcll(m);
+			for (int jx = 0; jx < APT; ++jx) {
+				const uint32_t k = ts + (uint32_t)jx * WG + tid;
+				const uint32_t o = (uint32_t)jx * WG + tid;
+				if (k < te) {
+					const uint32_t q = seg_find(s_k0, 0u, m, k);
+					const uint32_t r = k - s_k0[q];
+					Ag a;
+					if (s_pb[q] == PB_NEWBORN) newborn(a, T.nbseed, T.ip.nb_temp, s_vb[q] + r);
+					else load_agent(a, T.bin, P, s_pb[q] + r);
+					s_out[F_AGE][o] = sc_bits_f(a.age);
+					s_out[F_HIS][o] = sc_bits_f(a.his);
+					s_out[F_DLY][o] = sc_bits_f(a.dly);
+					s_out[F_AUX][o] = a.aux;
+					s_out[F_CSD][o] = sc_bits_f(a.csd);
+					s_out[F_AVAIL][o] = a.avail;
+					s_out[F_CYC][o] = a.cyc;
+					s_out[F_MISC][o] = a.misc;
+				}
 			}
+#pragma unroll 1
+			for (int jx = 0; jx < APT; ++jx) {
+				const uint32_t k = ts + (uint32_t)jx * WG + tid;
+				uint32_t n2 = S_DEAD;
+				if (k < te) {
+					const uint32_t q = seg_find(s_k0, 0u, m, k);
+					const uint32_t s = s_st[q];
+					const uint32_t g = s_vb[q] + (k - s_k0[q]);
+					const uint32_t o = (uint32_t)jx * WG + tid;
+					Ag a;
+					a.age = sc_float(s_out[F_AGE][o]);
+					a.his = sc_float(s_out[F_HIS][o]);
+					a.dly = sc_float(s_out[F_DLY][o]);
+					a.aux = s_out[F_AUX][o];
+					a.csd = sc_float(s_out[F_CSD][o]);
+					a.avail = s_out[F_AVAIL][o];
+					a.cyc = s_out[F_CYC][o];
+					a.misc = s_out[F_MISC][o];
+					n2 = agent_step(a, s, g, T, acc);
+					if (n2 < S_DEAD) {
+						s_out[F_AGE][o] = sc_bits_f(a.age);
+						s_out[F_HIS][o] = sc_bits_f(a.his);
+						s_out[F_DLY][o] = sc_bits_f(a.dly);
+						s_out[F_AUX][o] = a.aux;
+						s_out[F_CSD][o] = sc_bits_f(a.csd);
+						s_out[F_AVAIL][o] = a.avail;
+						s_out[F_CYC][o] = a.cyc;
+						s_out[F_MISC][o] = a.misc;
+						if (n2 == S_LARVA) l1de += sc_f2u(a.age / 24.0f);
+						if (n2 >= S_IMM && (a.misc >> 31)) {
+							fem += 1u;
+							pinf += a.csd >= 1.0f ? 1u : 0u;
+						}
+					}
+				}
+				/* rank survivors by new state */
+				uint32_t rank = 0;
+#pragma unroll
+				for (uint32_t st = 0; st < 8u; ++st) {
+					uint64_t mk = __ballot(n2 == st);
+					if (n2 == st) rank = (uint32_t)__popcll(mk & lt);
+					if (lane == 0) s_wc[jx][w][st] = (uint32_t)__popcll(mk);
+				}
+				ns[jx] = n2;
+				lr[jx] = rank;
+			}
+			__syncthreads();
+			if (tid < 8u) { /* exclusive over (jx, wave) per state, after the chunk's earlier tiles */
+				uint32_t run = s_run[tid];
+				for (int jx = 0; jx < APT; ++jx)
+					for (int q = 0; q < NW; ++q) {
+						uint32_t cc = s_wc[jx][q][tid];
+						s_wc[jx][q][tid] = run;
+						run += cc;
+					}
+				s_tc[tid] = run;
+			}
+			__syncthreads();
+			/* the segment holding te - 1 starts the next tile's lookup */
+			next = wlo + seg_find(s_k0, 0u, m, te - 1u);
+#pragma unroll 1
+			for (int jx = 0; jx < APT; ++jx) {
+				const uint32_t st = ns[jx];
+				if (st < S_DEAD) {
+					const uint32_t o = (uint32_t)jx * WG + tid;
+					const uint32_t p = T.ch.pout[st] + s_wc[jx][w][st] + lr[jx];
+					uint32_t *b = T.bout;
+					b[p] = s_out[F_AGE][o];
+					b[P + p] = s_out[F_HIS][o];
+					b[2 * P + p] = s_out[F_DLY][o];
+					b[3 * P + p] = s_out[F_AUX][o];
+					b[4 * P + p] = s_out[F_CSD][o];
+					b[5 * P + p] = s_out[F_AVAIL][o];
+					b[6 * P + p] = s_out[F_CYC][o];
+					b[7 * P + p] = s_out[F_MISC][o];
+				}
+			}
+			__syncthreads();
+			if (tid < 8u) s_run[tid] = s_tc[tid];
 		}
-		{
+		{ /* chunk results */
 			uint32_t v[8] = {acc.eggs, acc.bites, acc.ibites, acc.ncyc, acc.scyc, l1de, fem, pinf};
 #pragma unroll
 			for (int q = 0; q < 8; ++q) {
 				uint32_t r = wave_sum(v[q]);
 				if (lane == 0) s_acc[w][q] = r;
 			}
-		}
-		__syncthreads();
-		if (tid < 8u) { /* exclusive scan over (j, wave) per state */
-			uint32_t run = 0;
-			for (int j = 0; j < APT; ++j)
-				for (int q = 0; q < NW; ++q) {
-					uint32_t c = s_wc[j][q][tid];
-					s_wc[j][q][tid] = run;
-					run += c;
-				}
-			s_agg[tid] = run;
-		} else if (tid < 16u) {
-			uint32_t r = 0;
-			for (int q = 0; q < NW; ++q) r += s_acc[q][tid - 8u];
-			s_agg[tid] = r;
-		}
-		__syncthreads();
-		if (w == 0) { /* decoupled look-back over this instance's tiles */
-			uint64_t *mine = lb + (size_t)ticket * NV;
-			const uint32_t v = lane & 15u;
-			uint32_t excl = 0;
-			if (ticket == first) {
-				if (lane < 16u) lb_store(mine + lane, LB_INC, epoch, s_agg[lane]);
-			} else {
-				if (lane < 16u) lb_store(mine + lane, LB_AGG, epoch, s_agg[lane]);
-				uint32_t resolved = 0;
-				int64_t look = (int64_t)ticket - 1;
-				const uint32_t dpt = lane >> 4;
-				for (;;) {
-					uint32_t res_v = __shfl(resolved, (int)v, 64);
-					int64_t tt = look - (int64_t)dpt;
-					uint32_t kind = 2u, val = 0u;
-					if (!res_v && tt >= (int64_t)first) {
-						const uint64_t *pw = lb + (size_t)tt * NV + v;
-						uint64_t word;
-						for (;;) {
-							word = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-							if ((uint32_t)((word >> 32) & 0x3fffffffu) == epoch && (word >> 62) != 0ull) break;
-							__builtin_amdgcn_s_sleep(1);
-						}
-						kind = (uint32_t)(word >> 62);
-						val = (uint32_t)word;
-					}
-					uint32_t sum = 0, found = 0;
-#pragma unroll
-					for (int dd = 0; dd < 4; ++dd) {
-						uint32_t kv = __shfl(kind, dd * 16 + (int)v, 64);
-						uint32_t vv = __shfl(val, dd * 16 + (int)v, 64);
-						if (!found) {
-							sum += vv;
-							if (kv == 2u) found = 1;
-						}
-					}
-					if (!resolved) {
-						excl += sum;
-						resolved = found;
-					}
-					uint64_t open = __ballot(lane < 16u && !resolved);
-					if (open == 0ull) break;
-					look -= 4;
-				}
-				if (lane < 16u) lb_store(mine + lane, LB_INC, epoch, excl + s_agg[lane]);
+			__syncthreads();
+			if (tid < 8u) {
+				uint32_t r = 0;
+				for (int q = 0; q < NW; ++q) r += s_acc[q][tid];
+				T.cout[(size_t)(8u + tid) * T.cmax + c] = r;
+				T.cout[(size_t)tid * T.cmax + c] = s_run[tid];
 			}
-			if (lane < 16u) s_excl[lane] = excl;
-		}
-		__syncthreads();
-		/* scatter survivors to their final slots */
-#pragma unroll
-		for (int j = 0; j < APT; ++j) {
-			const uint32_t s = ns[j];
-			if (s < S_DEAD) {
-				uint32_t dst = T.ip.pout[s] + s_excl[s] + s_wc[j][w][s] + lr[j];
-				store_agent(a[j], T.bout, P, dst);
-			}
-		}
-		if (tl == ntile_i - 1u && tid < NV) { /* instance totals, for k_prep */
-			I->tot[tid] = s_excl[tid] + s_agg[tid];
-			if (tid == 0) I->pending = 1u;
+			if (tid == 0) insts[ii].pending = 1u;
 		}
 		__syncthreads();
 	}
@@ -608,14 +977,25 @@ __global__ void __launch_bounds__(WG) k_pass(Inst *insts, Ctx *ctx, const uint32
 
 /* ------------------------------------------------------------ export, histogram */
 
-__global__ void k_export(const uint32_t *buf, uint32_t P, sampo_iter ip, const uint32_t *nbz, sampo_agent *out)
+__device__ static inline uint32_t seg_find_global(const Seg *seg, uint32_t nseg, uint32_t k)
+{
+	uint32_t lo = 0, hi = nseg;
+	while (hi - lo > 1u) {
+		uint32_t mid = (lo + hi) >> 1;
+		if (seg[mid].k0 <= k) lo = mid; else hi = mid;
+	}
+	return lo;
+}
+
+__global__ void k_export(const uint32_t *buf, uint32_t P, const Seg *seg, uint32_t nseg, uint32_t n,
+                         const uint32_t *nbz, float nb_temp, sampo_agent *out)
 {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-	if (k >= ip.n) return;
-	uint32_t s = range_of(ip.kstart, k);
+	if (k >= n) return;
+	const Seg sg = seg[seg_find_global(seg, nseg, k)];
 	Ag a;
-	if (s == S_EGG && k < ip.enew) newborn(a, nbz, ip.nb_temp, k + ip.voff[s]);
-	else load_agent(a, buf, P, k + ip.off_in[s]);
+	if (sg.pbase == PB_NEWBORN) newborn(a, nbz, nb_temp, sg.vbase + (k - sg.k0));
+	else load_agent(a, buf, P, sg.pbase + (k - sg.k0));
 	sampo_agent r;
 	r.ageInHours = a.age;
 	r.hoursInState = a.his;
@@ -631,15 +1011,15 @@ __global__ void k_export(const uint32_t *buf, uint32_t P, sampo_iter ip, const u
 }
 
 /* ageHistogram.cl:13-35: LDS-private bins, one global atomic per bin per group */
-__global__ void __launch_bounds__(256) k_histogram(const uint32_t *buf, uint32_t P, sampo_iter ip, uint32_t *hist)
+__global__ void __launch_bounds__(256) k_histogram(const uint32_t *buf, const Seg *seg, uint32_t nseg, uint32_t k0,
+                                                   uint32_t n, uint32_t *hist)
 {
 	__shared__ uint32_t bins[100];
 	for (uint32_t q = threadIdx.x; q < 100u; q += blockDim.x) bins[q] = 0u;
 	__syncthreads();
-	const uint32_t k0 = ip.kstart[S_IMM];
-	for (uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x; k < ip.n; k += gridDim.x * blockDim.x) {
-		uint32_t s = range_of(ip.kstart, k);
-		float age = sc_float(buf[k + ip.off_in[s]]);
+	for (uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		const Seg sg = seg[seg_find_global(seg, nseg, k)];
+		float age = sc_float(buf[sg.pbase + (k - sg.k0)]);
 		uint32_t b = sc_f2u(age / 24.0f);
 		atomicAdd(&bins[b < 99u ? b : 99u], 1u);
 	}
@@ -669,29 +1049,31 @@ int fail(int code, const char *fmt, const char *a = "", long b = 0)
 
 struct Slot {
 	bool used = false;
-	uint32_t *buf = nullptr;
-	uint32_t *seeds = nullptr;
-	float *temp = nullptr;
-	sampo_env *env = nullptr;
-	sampo_row *rows = nullptr;
+	void *mem = nullptr; /* one allocation: buffers, segments, chunks, inputs */
 	uint32_t horizon = 0;
 	uint32_t P = 0;
 };
 
-typedef void (*pass_fn)(Inst *, Ctx *, const uint32_t *, const uint8_t *, uint32_t, uint64_t *);
+typedef void (*pass_fn)(Inst *, Ctx *, const uint32_t *, const uint8_t *, uint32_t);
 
-template <int WG, int APT> pass_fn pass_ptr() { return k_pass<WG, APT>; }
+template <int WG, int APT, int WAVES> pass_fn pass_ptr() { return k_pass<WG, APT, WAVES>; }
 
-pass_fn pick_pass(uint32_t wg, uint32_t apt)
+/* compiled tile shapes: work-group size, agents per thread, minimum waves
+ * per SIMD (0: no register cap) */
+pass_fn pick_pass(uint32_t wg, uint32_t apt, uint32_t waves)
 {
-#define SAMPO_TILE(W, A) if (wg == W && apt == A) return pass_ptr<W, A>();
-	SAMPO_TILE(64, 4) SAMPO_TILE(64, 8)
-	SAMPO_TILE(128, 1) SAMPO_TILE(128, 2) SAMPO_TILE(128, 4)
-	SAMPO_TILE(256, 1) SAMPO_TILE(256, 2) SAMPO_TILE(256, 4)
-	SAMPO_TILE(512, 1) SAMPO_TILE(512, 2)
+#define SAMPO_TILE(W, A, V) if (wg == W && apt == A && waves == V) return pass_ptr<W, A, V>();
+	SAMPO_TILE(64, 4, 0)
+	SAMPO_TILE(128, 1, 0) SAMPO_TILE(128, 2, 0) SAMPO_TILE(128, 4, 0)
+	SAMPO_TILE(256, 1, 0) SAMPO_TILE(256, 2, 0) SAMPO_TILE(256, 4, 0)
+	SAMPO_TILE(256, 1, 6) SAMPO_TILE(256, 1, 8) SAMPO_TILE(256, 2, 6)
+	SAMPO_TILE(512, 1, 0) SAMPO_TILE(512, 2, 0) SAMPO_TILE(512, 1, 6) SAMPO_TILE(512, 1, 8)
+	SAMPO_TILE(1024, 1, 0) SAMPO_TILE(1024, 1, 6)
 #undef SAMPO_TILE
 	return nullptr;
 }
+
+size_t align256(size_t v) { return (v + 255u) & ~(size_t)255u; }
 
 } // namespace
 
@@ -703,10 +1085,10 @@ struct sampo_engine {
 	Ctx *d_ctx = nullptr;
 	uint32_t *d_pl_base = nullptr;
 	uint8_t *d_pl_par = nullptr;
-	uint64_t *d_lb = nullptr;
-	size_t lb_tiles = 0;
 	std::vector<Slot> slots;
-	uint32_t wg = 256, apt = 4;
+	uint32_t wg = 512, apt = 1, waves = 6;
+	uint32_t chunk = 4096;
+	uint32_t target_chunks = 0; /* 0: 4 per resident work-group of the pass */
 	pass_fn pass = nullptr;
 	int grid = 0;
 	hipGraphExec_t day = nullptr;
@@ -740,25 +1122,6 @@ static int configure_pass(sampo_engine *e)
 	return 0;
 }
 
-/* look-back words for the largest tile count the admitted instances can reach */
-static int ensure_lb(sampo_engine *e)
-{
-	size_t tiles = 1;
-	uint32_t tile = e->wg * e->apt;
-	for (auto &s : e->slots)
-		if (s.used) tiles += (s.P + tile - 1u) / tile + 1u;
-	if (tiles <= e->lb_tiles) return 0;
-	HIPCHK(hipStreamSynchronize(e->stream));
-	drop_graph(e);
-	if (e->d_lb) HIPCHK(hipFree(e->d_lb));
-	e->d_lb = nullptr;
-	size_t words = tiles * NV;
-	HIPCHK(hipMalloc(&e->d_lb, words * sizeof(uint64_t)));
-	HIPCHK(hipMemset(e->d_lb, 0, words * sizeof(uint64_t)));
-	e->lb_tiles = tiles;
-	return 0;
-}
-
 extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine **out)
 {
 	if (!out || max_slots == 0 || max_slots > (1u << 20)) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
@@ -781,10 +1144,8 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 	memset(blank.data(), 0, sizeof(Inst) * max_slots);
 	for (auto &b : blank) b.status = SAMPO_EMPTY;
 	HIPCHK(hipMemcpy(e->d_inst, blank.data(), sizeof(Inst) * max_slots, hipMemcpyHostToDevice));
-	e->pass = pick_pass(e->wg, e->apt);
+	e->pass = pick_pass(e->wg, e->apt, e->waves);
 	int rc = configure_pass(e);
-	if (rc) { sampo_engine_destroy(e); return rc; }
-	rc = ensure_lb(e);
 	if (rc) { sampo_engine_destroy(e); return rc; }
 	*out = e;
 	return 0;
@@ -792,11 +1153,7 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 
 static void free_slot(Slot &s)
 {
-	if (s.buf) (void)hipFree(s.buf);
-	if (s.seeds) (void)hipFree(s.seeds);
-	if (s.temp) (void)hipFree(s.temp);
-	if (s.env) (void)hipFree(s.env);
-	if (s.rows) (void)hipFree(s.rows);
+	if (s.mem) (void)hipFree(s.mem);
 	s = Slot();
 }
 
@@ -811,28 +1168,32 @@ extern "C" void sampo_engine_destroy(sampo_engine *e)
 	(void)hipFree(e->d_ctx);
 	(void)hipFree(e->d_pl_base);
 	(void)hipFree(e->d_pl_par);
-	(void)hipFree(e->d_lb);
 	if (e->stream) (void)hipStreamDestroy(e->stream);
 	delete e;
 }
 
-extern "C" int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt)
+extern "C" int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt, uint32_t waves)
 {
 	if (!e) return fail(SAMPO_E_ARG, "null engine%s%ld");
-	pass_fn f = pick_pass(wg, apt);
+	pass_fn f = pick_pass(wg, apt, waves);
 	if (!f) return fail(SAMPO_E_ARG, "tile shape not compiled in%s%ld");
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
 	drop_graph(e);
 	e->wg = wg;
 	e->apt = apt;
+	e->waves = waves;
 	e->pass = f;
-	int rc = configure_pass(e);
-	if (rc) return rc;
-	return ensure_lb(e);
+	return configure_pass(e);
 }
 
-static uint32_t next_start(uint32_t end) { return ((end + SAMPO_PAD) / SAMPO_PAD) * SAMPO_PAD; }
+extern "C" int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chunks)
+{
+	if (!e || agents < 64u || agents > (1u << 26)) return fail(SAMPO_E_ARG, "chunk size must be 64..2^26%s%ld");
+	e->chunk = agents;
+	e->target_chunks = target_chunks;
+	return 0;
+}
 
 extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uint32_t *slot_out)
 {
@@ -852,58 +1213,6 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		if (!e->slots[i].used) { id = i; break; }
 	if (id == e->max_slots) return fail(SAMPO_E_FULL, "no free slot%s%ld");
 
-	/* ---- initial layout (src/abms.c:185-228, or the given population) ---- */
-	sampo_prep_in in;
-	memset(&in, 0, sizeof(in));
-	std::vector<uint32_t> soa;
-	if (sc->mode == SAMPO_MODE_REFERENCE) {
-		if ((uint64_t)sc->n_eggs > sc->max_agents) return fail(SAMPO_E_ARG, "n_eggs exceeds max_agents%s%ld");
-		in.cnt[0] = sc->n_eggs;
-		in.enew = sc->n_eggs;
-		in.vstart[0] = 0;
-		for (int s = 1; s < 8; ++s) in.vstart[s] = sc->n_eggs + 1u;
-	} else {
-		if (!sc->init_layout || (!sc->init_agents)) return fail(SAMPO_E_ARG, "config mode needs an initial population%s%ld");
-		const sampo_layout *L = sc->init_layout;
-		uint64_t pos = 0, k = 0;
-		if (L->start[0] != 0) return fail(SAMPO_E_ARG, "egg range must start at 0%s%ld");
-		for (int s = 0; s < 8; ++s) {
-			if (L->end[s] < L->start[s]) return fail(SAMPO_E_ARG, "range %s%ld has end < start", "", s);
-			if (s > 0 && L->start[s] <= L->end[s - 1]) return fail(SAMPO_E_ARG, "range %s%ld needs a gap before it", "", s);
-			in.cnt[s] = L->end[s] - L->start[s];
-			in.vstart[s] = L->start[s];
-			in.phys_in[s] = (uint32_t)pos;
-			pos += ((in.cnt[s] + SAMPO_PAD - 1u) / SAMPO_PAD) * SAMPO_PAD;
-		}
-		if (pos > P) return fail(SAMPO_E_ARG, "initial population exceeds physical slots%s%ld");
-		soa.assign((size_t)NFIELD * pos, 0u);
-		for (int s = 0; s < 8; ++s) {
-			for (uint32_t r = 0; r < in.cnt[s]; ++r, ++k) {
-				const sampo_agent &a = sc->init_agents[k];
-				size_t p = in.phys_in[s] + r;
-				soa[F_AGE * pos + p] = sc_bits_f(a.ageInHours);
-				soa[F_HIS * pos + p] = sc_bits_f(a.hoursInState);
-				soa[F_DLY * pos + p] = sc_bits_f(a.delay);
-				soa[F_AUX * pos + p] = a.aux;
-				soa[F_CSD * pos + p] = sc_bits_f(a.csd);
-				soa[F_AVAIL * pos + p] = a.availableEggs;
-				soa[F_CYC * pos + p] = a.cycleLength;
-				soa[F_MISC * pos + p] = (a.humanBloodmealCount & 0xffffu) | ((a.numEggBatches & 0x7fffu) << 16) |
-				                        (a.isFemale ? 0x80000000u : 0u);
-				if (s == S_LARVA) in.l1de += sc_f2u(a.ageInHours / 24.0f);
-				if (s >= S_IMM && a.isFemale) {
-					in.females += 1u;
-					in.potinf += a.csd >= 1.0f ? 1u : 0u;
-				}
-			}
-		}
-	}
-	in.step = 0;
-	in.cur = 0;
-	in.temp_t = sc->temp[0];
-	in.temp_prev = sc->temp[0];
-	in.env_t = sc->env[0];
-
 	Inst h;
 	memset(&h, 0, sizeof(h));
 	h.P = P;
@@ -911,54 +1220,123 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	h.mode = sc->mode;
 	h.env_count = sc->env_count;
 	h.max_agents = sc->max_agents;
+	h.chunk_size = e->chunk;
+	h.target_chunks = e->target_chunks ? e->target_chunks : 4u * (uint32_t)e->grid;
+	h.cmax = (uint32_t)((P64 + e->chunk - 1u) / e->chunk) + 2u;
+	h.smax = 8u * h.cmax + 2u;
 	for (int c = 0; c < 4; ++c) h.hmr_stage[c] = sm_hmr(sc->dmr[c]);
 	for (uint32_t d = 0; d < 100; ++d) h.hmr_adult[d] = sc->mode == SAMPO_MODE_CONFIG ? h.hmr_stage[3] : sm_hmr_adult(d);
-	sampo_row row0;
-	h.status = sm_prep(h.mode, h.horizon, h.max_agents, P, h.hmr_stage, &in, &h.p[0], &row0);
-	for (uint32_t d = 0; d < 100; ++d)
-		h.p[0].hmr_larva[d] = sc->mode == SAMPO_MODE_CONFIG ? h.hmr_stage[1]
-		                                                    : sm_hmr_larva(d, in.l1de, in.env_t.carryingCapacity);
-	h.step = 0;
+	IterState &s0 = h.st[0];
 
-	/* ---- device allocations and uploads ---- */
+	/* initial population (src/abms.c:185-228, or the given one) and its segments */
+	std::vector<Seg> segs;
+	std::vector<uint32_t> soa;
+	size_t stored = 0;
+	if (sc->mode == SAMPO_MODE_REFERENCE) {
+		if ((uint64_t)sc->n_eggs > sc->max_agents) return fail(SAMPO_E_ARG, "n_eggs exceeds max_agents%s%ld");
+		s0.cnt[0] = sc->n_eggs;
+		s0.enew = sc->n_eggs;
+		for (int s = 1; s < 8; ++s) s0.vstart[s] = sc->n_eggs + 1u;
+		if (sc->n_eggs) segs.push_back(Seg{0u, 0u, PB_NEWBORN, 0u});
+	} else {
+		if (!sc->init_layout || !sc->init_agents) return fail(SAMPO_E_ARG, "config mode needs an initial population%s%ld");
+		const sampo_layout *L = sc->init_layout;
+		if (L->start[0] != 0) return fail(SAMPO_E_ARG, "egg range must start at 0%s%ld");
+		uint64_t k = 0;
+		for (int s = 0; s < 8; ++s) {
+			if (L->end[s] < L->start[s]) return fail(SAMPO_E_ARG, "range %s%ld has end < start", "", s);
+			if (s > 0 && L->start[s] <= L->end[s - 1]) return fail(SAMPO_E_ARG, "range %s%ld needs a gap before it", "", s);
+			s0.cnt[s] = L->end[s] - L->start[s];
+			s0.vstart[s] = L->start[s];
+			if (s0.cnt[s]) segs.push_back(Seg{(uint32_t)k, (uint32_t)s, (uint32_t)k, L->start[s]});
+			k += s0.cnt[s];
+		}
+		stored = (size_t)k;
+		if (stored > P) return fail(SAMPO_E_ARG, "initial population exceeds physical slots%s%ld");
+		soa.assign((size_t)NFIELD * stored, 0u);
+		k = 0;
+		for (int s = 0; s < 8; ++s) {
+			for (uint32_t r = 0; r < s0.cnt[s]; ++r, ++k) {
+				const sampo_agent &a = sc->init_agents[k];
+				soa[F_AGE * stored + k] = sc_bits_f(a.ageInHours);
+				soa[F_HIS * stored + k] = sc_bits_f(a.hoursInState);
+				soa[F_DLY * stored + k] = sc_bits_f(a.delay);
+				soa[F_AUX * stored + k] = a.aux;
+				soa[F_CSD * stored + k] = sc_bits_f(a.csd);
+				soa[F_AVAIL * stored + k] = a.availableEggs;
+				soa[F_CYC * stored + k] = a.cycleLength;
+				soa[F_MISC * stored + k] = (a.humanBloodmealCount & 0xffffu) | ((a.numEggBatches & 0x7fffu) << 16) |
+				                           (a.isFemale ? 0x80000000u : 0u);
+				/* calcStats reductions of the initial population (calcL1de.cl, calcGender.cl) */
+				if (s == S_LARVA) h.l1de += sc_f2u(a.ageInHours / 24.0f);
+				if (s >= S_IMM && a.isFemale) {
+					h.females += 1u;
+					h.potinf += a.csd >= 1.0f ? 1u : 0u;
+				}
+			}
+		}
+	}
+	s0.nseg = (uint32_t)segs.size();
+	s0.step = 0;
+	s0.cur = 0;
+	h.step = 0;
+	h.status = SAMPO_RUNNING;
+	h.needs_begin = 1u;
+
+	/* one device allocation per instance */
+	const size_t seeds_n = (size_t)(sc->horizon + 1u) * 16u;
+	size_t off = 0;
+	const size_t o_buf = off; off = align256(off + (size_t)2 * NFIELD * P * sizeof(uint32_t));
+	const size_t o_seg0 = off; off = align256(off + (size_t)h.smax * sizeof(Seg));
+	const size_t o_seg1 = off; off = align256(off + (size_t)h.smax * sizeof(Seg));
+	const size_t o_ch0 = off; off = align256(off + (size_t)h.cmax * sizeof(Chunk));
+	const size_t o_ch1 = off; off = align256(off + (size_t)h.cmax * sizeof(Chunk));
+	const size_t o_cout = off; off = align256(off + (size_t)NOUT * h.cmax * sizeof(uint32_t));
+	const size_t o_seeds = off; off = align256(off + seeds_n * sizeof(uint32_t));
+	const size_t o_temp = off; off = align256(off + (size_t)sc->horizon * sizeof(float));
+	const size_t o_env = off; off = align256(off + (size_t)sc->env_count * sizeof(sampo_env));
+	const size_t o_rows = off; off = align256(off + (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
+
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
 	Slot S;
 	S.horizon = sc->horizon;
 	S.P = P;
-	std::vector<uint32_t> seeds((size_t)(sc->horizon + 1u) * 16u);
-	sampo_seed_stream(sc->seed, (uint32_t)seeds.size(), seeds.data());
-	hipError_t err = hipMalloc(&S.buf, (size_t)2 * NFIELD * P * sizeof(uint32_t));
-	if (err == hipSuccess) err = hipMalloc(&S.seeds, seeds.size() * sizeof(uint32_t));
-	if (err == hipSuccess) err = hipMalloc(&S.temp, (size_t)sc->horizon * sizeof(float));
-	if (err == hipSuccess) err = hipMalloc(&S.env, (size_t)sc->env_count * sizeof(sampo_env));
-	if (err == hipSuccess) err = hipMalloc(&S.rows, (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
+	hipError_t err = hipMalloc(&S.mem, off);
+	if (err != hipSuccess) return fail(SAMPO_E_NOMEM, "device allocation failed: %s%ld", hipGetErrorString(err));
+	char *base = (char *)S.mem;
+	std::vector<uint32_t> seeds(seeds_n);
+	sampo_seed_stream(sc->seed, (uint32_t)seeds_n, seeds.data());
+	h.buf[0] = (uint32_t *)(base + o_buf);
+	h.buf[1] = h.buf[0] + (size_t)NFIELD * P;
+	h.seg[0] = (Seg *)(base + o_seg0);
+	h.seg[1] = (Seg *)(base + o_seg1);
+	h.chunk[0] = (Chunk *)(base + o_ch0);
+	h.chunk[1] = (Chunk *)(base + o_ch1);
+	h.cout = (uint32_t *)(base + o_cout);
+	h.seeds = (const uint32_t *)(base + o_seeds);
+	h.temp = (const float *)(base + o_temp);
+	h.env = (const sampo_env *)(base + o_env);
+	h.rows = (sampo_row *)(base + o_rows);
+	err = hipMemcpy((void *)h.seeds, seeds.data(), seeds_n * sizeof(uint32_t), hipMemcpyHostToDevice);
+	if (err == hipSuccess) err = hipMemcpy((void *)h.temp, sc->temp, (size_t)sc->horizon * sizeof(float), hipMemcpyHostToDevice);
+	if (err == hipSuccess) err = hipMemcpy((void *)h.env, sc->env, (size_t)sc->env_count * sizeof(sampo_env), hipMemcpyHostToDevice);
+	if (err == hipSuccess) err = hipMemset(h.rows, 0, (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
+	if (err == hipSuccess && !segs.empty()) err = hipMemcpy(h.seg[0], segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice);
+	for (int f = 0; f < NFIELD && err == hipSuccess && stored; ++f)
+		err = hipMemcpy(h.buf[0] + (size_t)f * P, soa.data() + (size_t)f * stored, stored * sizeof(uint32_t),
+		                hipMemcpyHostToDevice);
+	if (err == hipSuccess) err = hipMemcpy(e->d_inst + id, &h, sizeof(Inst), hipMemcpyHostToDevice);
 	if (err != hipSuccess) {
 		free_slot(S);
-		return fail(SAMPO_E_NOMEM, "device allocation failed: %s%ld", hipGetErrorString(err));
+		return fail(SAMPO_E_HIP, "upload failed: %s%ld", hipGetErrorString(err));
 	}
-	HIPCHK(hipMemcpy(S.seeds, seeds.data(), seeds.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-	HIPCHK(hipMemcpy(S.temp, sc->temp, (size_t)sc->horizon * sizeof(float), hipMemcpyHostToDevice));
-	HIPCHK(hipMemcpy(S.env, sc->env, (size_t)sc->env_count * sizeof(sampo_env), hipMemcpyHostToDevice));
-	HIPCHK(hipMemset(S.rows, 0, (size_t)(sc->horizon + 1u) * sizeof(sampo_row)));
-	HIPCHK(hipMemcpy(S.rows, &row0, sizeof(row0), hipMemcpyHostToDevice));
-	if (!soa.empty()) {
-		size_t pos = soa.size() / NFIELD;
-		for (int f = 0; f < NFIELD; ++f)
-			HIPCHK(hipMemcpy(S.buf + (size_t)f * P, soa.data() + (size_t)f * pos, pos * sizeof(uint32_t),
-			                 hipMemcpyHostToDevice));
-	}
-	h.buf[0] = S.buf;
-	h.buf[1] = S.buf + (size_t)NFIELD * P;
-	h.seeds = S.seeds;
-	h.temp = S.temp;
-	h.env = S.env;
-	h.rows = S.rows;
-	HIPCHK(hipMemcpy(e->d_inst + id, &h, sizeof(Inst), hipMemcpyHostToDevice));
+	/* begin iteration 0 on the device (calcStats of src/abms.c:503) */
+	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipStreamSynchronize(e->stream));
 	S.used = true;
 	e->slots[id] = S;
-	int rc = ensure_lb(e);
-	if (rc) return rc;
 	*slot_out = id;
 	return 0;
 }
@@ -974,15 +1352,23 @@ extern "C" int sampo_engine_release(sampo_engine *e, uint32_t slot)
 	return 0;
 }
 
+static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after_pass)
+{
+	hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
+	                   e->max_slots);
+	if (before_pass) HIPCHK(hipEventRecord(before_pass, e->stream));
+	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots};
+	HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
+	if (after_pass) HIPCHK(hipEventRecord(after_pass, e->stream));
+	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst);
+	return 0;
+}
+
 static int enqueue(sampo_engine *e, uint32_t n)
 {
-	const uint32_t tile = e->wg * e->apt;
 	for (uint32_t i = 0; i < n; ++i) {
-		hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
-		                   e->d_pl_par, e->max_slots, tile);
-		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots, &e->d_lb};
-		HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
-		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(128), 0, e->stream, e->d_inst);
+		int rc = enqueue_one(e, nullptr, nullptr);
+		if (rc) return rc;
 	}
 	HIPCHK(hipGetLastError());
 	return 0;
@@ -1059,18 +1445,17 @@ extern "C" int sampo_engine_status(sampo_engine *e, uint32_t slot, uint32_t *sta
 	if (rc) return rc;
 	if (status) *status = h.status;
 	if (step) *step = h.step;
-	if (n_agents) *n_agents = h.p[h.step & 1u].n;
+	if (n_agents) *n_agents = h.st[h.step & 1u].n;
 	return 0;
 }
 
 extern "C" int sampo_engine_rows(sampo_engine *e, uint32_t slot, uint32_t first, uint32_t n, sampo_row *out)
 {
-	if (!e || slot >= e->max_slots || !e->slots[slot].used || !out) return fail(SAMPO_E_ARG, "bad slot%s%ld");
-	const Slot &S = e->slots[slot];
-	if ((uint64_t)first + n > (uint64_t)S.horizon + 1u) return fail(SAMPO_E_ARG, "rows out of range%s%ld");
-	HIPCHK(hipSetDevice(e->device));
-	HIPCHK(hipStreamSynchronize(e->stream));
-	HIPCHK(hipMemcpy(out, S.rows + first, (size_t)n * sizeof(sampo_row), hipMemcpyDeviceToHost));
+	Inst h;
+	int rc = read_inst(e, slot, &h);
+	if (rc) return rc;
+	if (!out || (uint64_t)first + n > (uint64_t)h.horizon + 1u) return fail(SAMPO_E_ARG, "rows out of range%s%ld");
+	HIPCHK(hipMemcpy(out, h.rows + first, (size_t)n * sizeof(sampo_row), hipMemcpyDeviceToHost));
 	return 0;
 }
 
@@ -1080,7 +1465,8 @@ extern "C" int sampo_engine_export(sampo_engine *e, uint32_t slot, sampo_layout 
 	Inst h;
 	int rc = read_inst(e, slot, &h);
 	if (rc) return rc;
-	const sampo_iter &ip = h.p[h.step & 1u];
+	const uint32_t par = h.step & 1u;
+	const IterState &ip = h.st[par];
 	if (layout)
 		for (int s = 0; s < 8; ++s) {
 			layout->start[s] = ip.vstart[s];
@@ -1092,7 +1478,8 @@ extern "C" int sampo_engine_export(sampo_engine *e, uint32_t slot, sampo_layout 
 	sampo_agent *d = nullptr;
 	HIPCHK(hipMalloc(&d, (size_t)ip.n * sizeof(sampo_agent)));
 	const uint32_t *nbz = h.seeds + (size_t)ip.nb_set * 16u + 8u;
-	hipLaunchKernelGGL(k_export, dim3((ip.n + 255u) / 256u), dim3(256), 0, e->stream, h.buf[ip.cur], h.P, ip, nbz, d);
+	hipLaunchKernelGGL(k_export, dim3((ip.n + 255u) / 256u), dim3(256), 0, e->stream, h.buf[ip.cur], h.P, h.seg[par],
+	                   ip.nseg, ip.n, nbz, ip.nb_temp, d);
 	hipError_t err = hipGetLastError();
 	if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
 	if (err == hipSuccess) err = hipMemcpy(agents, d, (size_t)ip.n * sizeof(sampo_agent), hipMemcpyDeviceToHost);
@@ -1106,7 +1493,8 @@ extern "C" int sampo_engine_histogram(sampo_engine *e, uint32_t slot, uint32_t h
 	Inst h;
 	int rc = read_inst(e, slot, &h);
 	if (rc) return rc;
-	const sampo_iter &ip = h.p[h.step & 1u];
+	const uint32_t par = h.step & 1u;
+	const IterState &ip = h.st[par];
 	uint32_t *d = nullptr;
 	HIPCHK(hipMalloc(&d, 100 * sizeof(uint32_t)));
 	hipError_t err = hipMemsetAsync(d, 0, 100 * sizeof(uint32_t), e->stream);
@@ -1114,7 +1502,8 @@ extern "C" int sampo_engine_histogram(sampo_engine *e, uint32_t slot, uint32_t h
 	if (err == hipSuccess && adults > 0) {
 		uint32_t blocks = (adults + 255u) / 256u;
 		if (blocks > 2048u) blocks = 2048u;
-		hipLaunchKernelGGL(k_histogram, dim3(blocks), dim3(256), 0, e->stream, h.buf[ip.cur], h.P, ip, d);
+		hipLaunchKernelGGL(k_histogram, dim3(blocks), dim3(256), 0, e->stream, h.buf[ip.cur], h.seg[par], ip.nseg,
+		                   ip.kstart[S_IMM], ip.n, d);
 		err = hipGetLastError();
 	}
 	if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
@@ -1263,17 +1652,11 @@ extern "C" int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_m
 {
 	if (!e || !wall_ms || !pass_ms || n == 0) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
 	HIPCHK(hipSetDevice(e->device));
-	const uint32_t tile = e->wg * e->apt;
 	std::vector<hipEvent_t> ev(2 * (size_t)n);
 	for (auto &x : ev) HIPCHK(hipEventCreate(&x));
 	for (uint32_t i = 0; i < n; ++i) {
-		hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
-		                   e->d_pl_par, e->max_slots, tile);
-		HIPCHK(hipEventRecord(ev[2 * i], e->stream));
-		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots, &e->d_lb};
-		HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
-		HIPCHK(hipEventRecord(ev[2 * i + 1], e->stream));
-		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(128), 0, e->stream, e->d_inst);
+		int rc = enqueue_one(e, ev[2 * i], ev[2 * i + 1]);
+		if (rc) return rc;
 	}
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipEventSynchronize(ev[2 * (size_t)n - 1]));

@@ -123,9 +123,10 @@ def test_engine_tile_shapes_agree():
     steps = 24 * 4
     temp, env = ref_inputs(steps, 20000.0)
     ref = None
-    for wg, apt in ((256, 1), (128, 2), (64, 4), (512, 2), (256, 4)):
+    for wg, apt, waves in ((256, 1, 0), (128, 2, 0), (64, 4, 0), (512, 2, 0), (256, 4, 0), (512, 1, 6),
+                           (1024, 1, 0), (256, 1, 8)):
         with S.Engine(max_slots=1) as e:
-            e.set_tile(wg, apt)
+            e.set_tile(wg, apt, waves)
             s = e.admit(seed=9, horizon=steps, temp=temp, env=env, n_eggs=5000, max_agents=1_000_000)
             e.step(steps)
             got = (e.rows(s, 0, steps + 1), e.export(s)[1])
