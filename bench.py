@@ -114,8 +114,11 @@ def main():
     wg, apt, waves = (int(x) for x in (args.tile.split("x") + ["0"])[:3])
     with S.Engine(max_slots=1) as e:
         e.set_tile(wg, apt, waves)
+        t_admit = time.perf_counter()
         slot = e.admit(mode=S.MODE_CONFIG, seed=w["seed"], horizon=24 * w["days"], temp=temp, env=env,
                        max_agents=w["max_agents"], dmr=DMR, layout=layout, agents=agents)
+        e.sync()
+        admit_ms = (time.perf_counter() - t_admit) * 1e3
         del agents
         if args.warmup:
             e.step(24 * args.warmup)
@@ -154,6 +157,7 @@ def main():
                      "kernel": "k_pass", "launches": n_iter, "algorithmic_bytes_per_launch": per_launch,
                      "mean_launch_ms": pass_ms / n_iter, "stream_copy_GBs": 2 * copy_bytes / (copy_ms / 1e3) / 1e9},
         "gpu_wall_ms_events": wall_ms,
+        "admit_ms": admit_ms,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_days, args.cpu_scale)

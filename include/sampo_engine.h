@@ -121,7 +121,7 @@ int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt, uint32_t w
 /* Chunking of instances admitted from now on.  A chunk is the run of agents
  * one work-group takes and the granularity of output segments: at least
  * `agents` agents (default 4096), grown so that an instance has at most
- * `target_chunks` chunks (0: four per resident work-group of the pass). */
+ * `target_chunks` chunks (0: two per resident work-group of the pass). */
 int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chunks);
 
 /* Admit a scenario into a free slot: allocates its device state, uploads its

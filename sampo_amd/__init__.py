@@ -175,7 +175,7 @@ class Engine:
 
     def set_chunk(self, agents, target_chunks=0):
         """Minimum agents per chunk (work unit / output segment) and the most
-        chunks per instance (0: four per resident work-group) for later
+        chunks per instance (0: two per resident work-group) for later
         admissions."""
         _check(lib().sampo_engine_set_chunk(self.h, agents, target_chunks))
 

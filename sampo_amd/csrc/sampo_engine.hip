@@ -105,14 +105,7 @@ struct Ctx {
 
 /* ------------------------------------------------------------ the agent */
 
-/* devRates, include/properties.h:94-98: hourly larval development, 12..43 degC */
-__constant__ float c_dev_rates[32] = {
-	0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f,
-	0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f,
-	0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f,
-	0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f,
-	0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f,
-	0.00000533086f, 0.0000000840986f};
+__constant__ float c_dev_rates[32] = SM_DEV_RATES_INIT;
 
 struct Ag {
 	float age, his, dly, csd;
@@ -190,10 +183,7 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 		if (ip.env.larvacideValue > 0.0f && a.his == 1.0f && rnd(&T.seed[8], g) <= ip.env.larvacideValue)
 			return S_DEAD;
 		/* larvaDevelopment, properties.h:105-109; index clamped to the table */
-		float tc = sc_minf(sc_maxf(t, 12.0f), 48.0f);
-		int ix = (int)tc - 12;
-		if (ix > 31) ix = 31;
-		float c = sc_float(a.aux) + c_dev_rates[ix];
+		float c = sc_float(a.aux) + c_dev_rates[sm_dev_index(t)];
 		a.aux = sc_bits_f(c);
 		if (c >= a.dly && night(h)) {
 			a.his = 0.0f;
@@ -517,10 +507,13 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		uint32_t n[B];
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u) {
-			uint32_t q = qb + u;
+			uint32_t q = qb + u < q1 ? qb + u : q1 - 1u;
 			uint32_t s = q / C, c = q - s * C;
-			n[u] = q < q1 ? co[(size_t)s * cmax + c] : 0u;
+			n[u] = co[(size_t)s * cmax + c];
 		}
+#pragma unroll
+		for (uint32_t u = 0; u < B; ++u)
+			if (qb + u >= q1) n[u] = 0u;
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u) {
 			sk += n[u];
@@ -533,12 +526,15 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	for (uint32_t qb = q0; qb < q1; qb += B) {
 		uint32_t n[B], pb[B];
 #pragma unroll
-		for (uint32_t u = 0; u < B; ++u) {
-			uint32_t q = qb + u < q1 ? qb + u : q0;
+		for (uint32_t u = 0; u < B; ++u) { /* unconditional loads: one wait per batch */
+			uint32_t q = qb + u < q1 ? qb + u : q1 - 1u;
 			uint32_t s = q / C, c = q - s * C;
-			n[u] = qb + u < q1 ? co[(size_t)s * cmax + c] : 0u;
+			n[u] = co[(size_t)s * cmax + c];
 			pb[u] = ch[c].pout[s];
 		}
+#pragma unroll
+		for (uint32_t u = 0; u < B; ++u)
+			if (qb + u >= q1) n[u] = 0u;
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u) {
 			if (!n[u]) continue;
@@ -1088,7 +1084,7 @@ struct sampo_engine {
 	std::vector<Slot> slots;
 	uint32_t wg = 512, apt = 1, waves = 6;
 	uint32_t chunk = 4096;
-	uint32_t target_chunks = 0; /* 0: 4 per resident work-group of the pass */
+	uint32_t target_chunks = 0; /* 0: 2 per resident work-group of the pass */
 	pass_fn pass = nullptr;
 	int grid = 0;
 	hipGraphExec_t day = nullptr;
@@ -1221,7 +1217,7 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	h.env_count = sc->env_count;
 	h.max_agents = sc->max_agents;
 	h.chunk_size = e->chunk;
-	h.target_chunks = e->target_chunks ? e->target_chunks : 4u * (uint32_t)e->grid;
+	h.target_chunks = e->target_chunks ? e->target_chunks : 2u * (uint32_t)e->grid;
 	h.cmax = (uint32_t)((P64 + e->chunk - 1u) / e->chunk) + 2u;
 	h.smax = 8u * h.cmax + 2u;
 	for (int c = 0; c < 4; ++c) h.hmr_stage[c] = sm_hmr(sc->dmr[c]);
@@ -1576,20 +1572,8 @@ extern "C" int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_
 	uint64_t x = seed ^ 0x504f50u;
 	std::vector<sampo_agent> by[4];
 	const int stage_of[4] = {S_EGG, S_LARVA, S_PUPA, S_IMM};
-	float tc = sc_minf(sc_maxf(t0, 12.0f), 48.0f);
-	int ix = (int)tc - 12;
-	if (ix > 31) ix = 31;
-	float dev;
-	{ /* devRates entry for t0 (host copy of the device table) */
-		static const float rates[32] = {
-			0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f,
-			0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f,
-			0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f,
-			0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f,
-			0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f,
-			0.00000533086f, 0.0000000840986f};
-		dev = rates[ix];
-	}
+	static const float rates[32] = SM_DEV_RATES_INIT;
+	const float dev = rates[sm_dev_index(t0)];
 	for (uint32_t i = 0; i < n0; ++i) {
 		double us = sm64_unit(&x);
 		int st = us < 0.3 ? 0 : us < 0.6 ? 1 : us < 0.7 ? 2 : 3;

@@ -11,16 +11,11 @@
 
 #include "sampo_canon.h"
 #include "sampo_icl_types.h"
+#include "sampo_model.h"
 
 #define LOCAL 64u /* LOCAL_SIZE, include/agent.h:16 */
 
-__constant__ float c_rates[32] = {
-	0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f,
-	0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f,
-	0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f,
-	0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f,
-	0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f,
-	0.00000533086f, 0.0000000840986f};
+__constant__ float c_rates[32] = SM_DEV_RATES_INIT;
 
 __device__ static inline uint32_t gid1() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -150,10 +145,7 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 			states[g] = st;
 			return;
 		}
-		float tc = sc_minf(sc_maxf(t, 12.0f), 48.0f);
-		int ix = (int)tc - 12;
-		if (ix > 31) ix = 31;
-		float cl = sc_float(a.aux) + c_rates[ix];
+		float cl = sc_float(a.aux) + c_rates[sm_dev_index(t)];
 		a.aux = sc_bits_f(cl);
 		if (cl >= a.delay && night(h)) {
 			ae.hoursInState = 0.0f;

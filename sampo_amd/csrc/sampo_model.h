@@ -35,6 +35,25 @@ enum { S_EGG = 0, S_LARVA, S_PUPA, S_IMM, S_MS, S_BMS, S_BMD, S_GRAVID, S_DEAD }
 
 #define SAMPO_PAD 64u  /* LOCAL_SIZE of the reference layout (include/agent.h:16) */
 
+/* devRates, include/properties.h:94-98: hourly larval development at
+ * 12, 13, ..., 43 degC; the one initializer every table in the library uses */
+#define SM_DEV_RATES_INIT { \
+	0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f, \
+	0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f, \
+	0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f, \
+	0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f, \
+	0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f, \
+	0.00000533086f, 0.0000000840986f}
+
+/* devRates index of larvaDevelopment (properties.h:105-109): T clamped to
+ * [12, 48], truncated, minus 12, and clamped to the 32-entry table */
+SC_FN int sm_dev_index(float t)
+{
+	float tc = sc_minf(sc_maxf(t, 12.0f), 48.0f);
+	int ix = (int)tc - 12;
+	return ix > 31 ? 31 : ix;
+}
+
 /* properties.h:29-31 and 39-60 */
 SC_FN float sm_hatching(float p)
 {
