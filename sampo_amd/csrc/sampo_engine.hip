@@ -42,25 +42,46 @@
 /* ------------------------------------------------------------ device state */
 
 #define PB_NEWBORN 0xffffffffu
+#define SEG_REV 0x100u /* in Seg.state: the run is stored backwards from pbase */
 
 struct Seg { /* a run of one state, stored contiguously (or the newborns) */
 	uint32_t k0;     /* dense index of its first agent */
-	uint32_t state;
+	uint32_t state;  /* state, | SEG_REV if agent k0 + r is at pbase - r */
 	uint32_t pbase;  /* physical slot of its first agent, PB_NEWBORN for newborns */
 	uint32_t vbase;  /* virtual index of its first agent */
 };
+
+__host__ __device__ static inline uint32_t seg_slot(uint32_t pbase, uint32_t state, uint32_t r)
+{
+	return (state & SEG_REV) ? pbase - r : pbase + r;
+}
 
 struct Chunk {
 	uint32_t k0, k1; /* dense range */
 	uint32_t seg0;   /* segment holding k0 */
 	uint32_t pad;
-	uint32_t pout[8]; /* output base per new state, in the other buffer */
 };
 
-/* per-chunk results, structure of arrays: cout[q * cmax + c] for
- * q < 8 survivors per new state, then newEggs, bites, infectious bites,
- * cycles, cycle sum, L1DE, females, potentially infective */
-#define NOUT 16
+/* Output layout.  The agents of chunk c that had state s at the start of the
+ * iteration occupy the dense range [head, end) of that state within the
+ * chunk; in the other buffer the same slot range receives their survivors:
+ * those that keep state s from `head` upwards, those that move on to the
+ * next state (s + 1, or BMS for gravid) from `end - 1` downwards, both in
+ * old-index order.  Deaths leave the middle empty.  The next iteration's
+ * state-s' range is, chunk by chunk, the run moved in from the previous
+ * state, the run that stayed, and for BMS the run moved in from gravid --
+ * the reference's stable order -- and an iteration needs exactly as many
+ * output slots as it has agents.
+ *
+ * Per-chunk results, structure of arrays, cout[q * cmax + c]: q < 8 agents of
+ * input state q that stayed, 8 + q agents of input state q that moved on,
+ * then newEggs, bites, infectious bites, cycles, cycle sum, L1DE, females,
+ * potentially infective. */
+#define NOUT 24
+#define CO_SUMS 16
+
+/* destination of a state's movers: the next state, gravid back to BMS */
+__host__ __device__ static inline uint32_t mover_dest(uint32_t s) { return s == S_GRAVID ? (uint32_t)S_BMS : s + 1u; }
 
 struct IterState {
 	uint32_t step, active, cur, n;
@@ -418,6 +439,28 @@ __device__ static inline uint32_t chunk_agents(const Inst *I, uint32_t N)
 	return cs;
 }
 
+/* Output runs of one chunk in the next iteration's dense order: for each new
+ * state s', the run moved in from the previous state, the run that stayed,
+ * and for BMS the run moved in from gravid -- 16 runs per chunk.  Run q of
+ * the list (new state, chunk, run) over C chunks: its new state, chunk and
+ * source (input state, moved). */
+__device__ static inline void run_of(uint32_t q, uint32_t C, uint32_t *sn_out, uint32_t *c_out, uint32_t *src,
+                                     uint32_t *moved)
+{
+	/* first run of new state s' within a chunk: 0, 1, 3, 5, 7, 9, 12, 14 (16 in all) */
+	const uint32_t sn = (q >= C) + (q >= 3u * C) + (q >= 5u * C) + (q >= 7u * C) + (q >= 9u * C) + (q >= 12u * C) +
+	                    (q >= 14u * C);
+	const uint32_t first = sn == 0u ? 0u : (sn <= 5u ? 2u * sn - 1u : 2u * sn);
+	const uint32_t nk = sn == 0u ? 1u : (sn == (uint32_t)S_BMS ? 3u : 2u);
+	const uint32_t rel = q - C * first, c = rel / nk, kind = rel - c * nk;
+	*sn_out = sn;
+	*c_out = c;
+	if (sn == 0u) { *src = 0u; *moved = 0u; }
+	else if (kind == 0u) { *src = sn - 1u; *moved = 1u; }
+	else if (kind == 1u) { *src = sn; *moved = 0u; }
+	else { *src = S_GRAVID; *moved = 1u; }
+}
+
 /* Stage A: fold the finished iteration of I (parity par) into the segment
  * list, ranges and sums of the next one (parity par ^ 1). */
 __device__ static void fold_iteration(Inst *I, uint32_t par)
@@ -429,7 +472,6 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	IterState &np = I->st[par ^ 1u];
 	const uint32_t C = ip.nchunk, cmax = I->cmax;
 	const uint32_t *co = I->cout;
-	const Chunk *ch = I->chunk[par];
 	Chunk *nch = I->chunk[par ^ 1u];
 	Seg *seg = I->seg[par ^ 1u];
 	if (tid < NOUT) s_tot[tid] = 0u;
@@ -453,11 +495,14 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		}
 	}
 	__syncthreads();
-	const uint32_t E = s_tot[8];
+	const uint32_t E = s_tot[CO_SUMS];
 	if (tid == 0) {
 		uint32_t k = 0, vend = 0;
-		for (int s = 0; s < 8; ++s) {
-			uint32_t c = s_tot[s] + (s == 0 ? E : 0u);
+		for (uint32_t s = 0; s < 8u; ++s) {
+			uint32_t c = s_tot[s];
+			if (s == 0u) c += E;
+			else c += s_tot[8u + s - 1u];
+			if (s == S_BMS) c += s_tot[8u + S_GRAVID];
 			np.cnt[s] = c;
 			s_kst[s] = k;
 			k += c;
@@ -470,13 +515,13 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		np.enew = E;
 		np.cur = ip.cur ^ 1u;
 		np.step = ip.step + 1u;
-		I->bites = s_tot[9];
-		I->ibites = s_tot[10];
-		I->ncyc = s_tot[11];
-		I->scyc = s_tot[12];
-		I->l1de = s_tot[13];
-		I->females = s_tot[14];
-		I->potinf = s_tot[15];
+		I->bites = s_tot[CO_SUMS + 1];
+		I->ibites = s_tot[CO_SUMS + 2];
+		I->ncyc = s_tot[CO_SUMS + 3];
+		I->scyc = s_tot[CO_SUMS + 4];
+		I->l1de = s_tot[CO_SUMS + 5];
+		I->females = s_tot[CO_SUMS + 6];
+		I->potinf = s_tot[CO_SUMS + 7];
 		const uint32_t cs = chunk_agents(I, k);
 		uint32_t nc = (uint32_t)(((uint64_t)k + cs - 1u) / cs);
 		s_cs = cs;
@@ -486,9 +531,10 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	}
 	__syncthreads();
 	const uint32_t cs = s_cs, NC = s_C;
-	/* segments: newborns, then (state, chunk) in state-major order; each
-	 * thread takes a contiguous run of (state, chunk) entries.  Each new
-	 * chunk's first segment is recorded as its start is passed. */
+	const uint32_t ocs = ip.cs, on = ip.n; /* chunking of the finished iteration */
+	/* segments: newborns, then the runs in (new state, chunk, run) order;
+	 * each thread takes a contiguous stretch of runs.  Each new chunk's first
+	 * segment is recorded as its start is passed. */
 	const uint32_t head = E > 0u ? 1u : 0u;
 	if (head) {
 		if (tid == 0) {
@@ -497,7 +543,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		}
 		for (uint32_t c = tid; c < NC && c * cs < E; c += PREP_WG) nch[c].seg0 = 0u;
 	}
-	const uint32_t total = 8u * C;
+	const uint32_t total = 16u * C;
 	const uint32_t L = (total + PREP_WG - 1u) / PREP_WG;
 	const uint32_t q0 = tid * L < total ? tid * L : total;
 	const uint32_t q1 = q0 + L < total ? q0 + L : total;
@@ -508,8 +554,9 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u) {
 			uint32_t q = qb + u < q1 ? qb + u : q1 - 1u;
-			uint32_t s = q / C, c = q - s * C;
-			n[u] = co[(size_t)s * cmax + c];
+			uint32_t sn, c, src, mv;
+			run_of(q, C, &sn, &c, &src, &mv);
+			n[u] = co[(size_t)(mv * 8u + src) * cmax + c];
 		}
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u)
@@ -524,13 +571,19 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	uint32_t k0 = E + block_excl(sk, lds, &tk);
 	uint32_t idx = head + block_excl(si, lds, &ti);
 	for (uint32_t qb = q0; qb < q1; qb += B) {
-		uint32_t n[B], pb[B];
+		uint32_t n[B], pb[B], stw[B];
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u) { /* unconditional loads: one wait per batch */
 			uint32_t q = qb + u < q1 ? qb + u : q1 - 1u;
-			uint32_t s = q / C, c = q - s * C;
-			n[u] = co[(size_t)s * cmax + c];
-			pb[u] = ch[c].pout[s];
+			uint32_t sn, c, src, mv;
+			run_of(q, C, &sn, &c, &src, &mv);
+			n[u] = co[(size_t)(mv * 8u + src) * cmax + c];
+			/* the source block [max(kstart[src], c*ocs), min(kstart[src+1], (c+1)*ocs, n)) */
+			const uint32_t ck0 = c * ocs, ck1 = ck0 + ocs < on ? ck0 + ocs : on;
+			const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
+			const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
+			pb[u] = mv ? be - 1u : bh;
+			stw[u] = sn | (mv ? SEG_REV : 0u);
 		}
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u)
@@ -538,13 +591,12 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 #pragma unroll
 		for (uint32_t u = 0; u < B; ++u) {
 			if (!n[u]) continue;
-			uint32_t q = qb + u;
-			uint32_t s = q / C;
+			const uint32_t sn = stw[u] & 0xffu;
 			Seg sg;
 			sg.k0 = k0;
-			sg.state = s;
+			sg.state = stw[u];
 			sg.pbase = pb[u];
-			sg.vbase = s_vst[s] + (k0 - s_kst[s]);
+			sg.vbase = s_vst[sn] + (k0 - s_kst[sn]);
 			seg[idx] = sg;
 			for (uint32_t nc = (k0 + cs - 1u) / cs; nc < NC && nc * cs < k0 + n[u]; ++nc) nch[nc].seg0 = idx;
 			++idx;
@@ -558,7 +610,6 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
  * printStats row, checks, per-iteration parameters and chunks. */
 __device__ static void begin_iteration(Inst *I, uint32_t par)
 {
-	__shared__ uint32_t lds[PREP_WG / 64];
 	__shared__ uint32_t s_status, s_go;
 	const uint32_t tid = threadIdx.x;
 	IterState &ip = I->st[par];
@@ -620,8 +671,8 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 	}
 	__syncthreads();
 	uint32_t st = s_status;
-	/* chunks: equal runs of chunk_size agents; outputs bounded by the inputs
-	 * that can reach each state in one iteration */
+	/* chunks: equal runs of chunk_size agents.  Their outputs go to the same
+	 * dense slot range of the other buffer, so N <= P is all the room needed. */
 	const uint32_t N = ip.n;
 	const uint32_t cs = chunk_agents(I, N);
 	const uint32_t C = st == SAMPO_RUNNING ? (N + cs - 1u) / cs : 0u;
@@ -629,45 +680,9 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 	Chunk *ch = I->chunk[par];
 	const Seg *seg = I->seg[par];
 	const uint32_t nseg = ip.nseg;
-	uint64_t carry = 0;
-	for (uint32_t b = 0; b < C; b += PREP_WG) {
-		uint32_t c = b + tid;
-		uint32_t size = 0;
-		uint32_t bound[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-		Chunk x;
-		if (c < C) {
-			x.k0 = c * cs;
-			x.k1 = x.k0 + cs < N ? x.k0 + cs : N;
-			uint32_t in[8];
-			for (int s = 0; s < 8; ++s) {
-				uint32_t a = ip.kstart[s] > x.k0 ? ip.kstart[s] : x.k0;
-				uint32_t z = ip.kstart[s + 1] < x.k1 ? ip.kstart[s + 1] : x.k1;
-				in[s] = z > a ? z - a : 0u;
-			}
-			bound[0] = in[0];
-			bound[1] = in[1] + in[0];
-			bound[2] = in[2] + in[1];
-			bound[3] = in[3] + in[2];
-			bound[4] = in[4] + in[3];
-			bound[5] = in[5] + in[4] + in[7];
-			bound[6] = in[6] + in[5];
-			bound[7] = in[7] + in[6];
-			for (int s = 0; s < 8; ++s) size += sm_roundup(bound[s], SAMPO_PAD);
-		}
-		uint32_t tot;
-		uint32_t base = block_excl(size, lds, &tot);
-		if (c < C) {
-			uint64_t p = carry + base;
-			for (int s = 0; s < 8; ++s) {
-				x.pout[s] = (uint32_t)p;
-				p += sm_roundup(bound[s], SAMPO_PAD);
-			}
-			/* seg0 was filled by the fold (or is filled below) */
-			ch[c].k0 = x.k0;
-			ch[c].k1 = x.k1;
-			for (int s = 0; s < 8; ++s) ch[c].pout[s] = x.pout[s];
-		}
-		carry += tot;
+	for (uint32_t c = tid; c < C; c += PREP_WG) {
+		ch[c].k0 = c * cs;
+		ch[c].k1 = c * cs + cs < N ? c * cs + cs : N;
 	}
 	__syncthreads();
 	/* initial populations: segment holding each chunk's first agent, segment
@@ -678,8 +693,9 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 			uint32_t z = j + 1u < nseg ? seg[j + 1u].k0 : N;
 			for (uint32_t c = (a + cs - 1u) / cs; c < C && c * cs < z; ++c) ch[c].seg0 = j;
 		}
+	__syncthreads();
 	if (tid == 0) {
-		if (st == SAMPO_RUNNING && carry > (uint64_t)I->P) st = SAMPO_PHYS_FULL;
+		ip.cs = cs;
 		ip.active = st == SAMPO_RUNNING;
 		ip.nchunk = ip.active ? C : 0u;
 		I->status = st;
@@ -749,8 +765,9 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	__shared__ uint32_t s_ticket, s_inst;
 	__shared__ uint32_t s_k0[SEGC], s_st[SEGC], s_pb[SEGC], s_vb[SEGC];
 	__shared__ uint32_t s_wlo, s_wm, s_wend; /* window: global index, size, dense end */
-	__shared__ uint32_t s_wc[APT][NW][8];
-	__shared__ uint32_t s_run[8], s_tc[8];
+	__shared__ uint32_t s_wc[APT][NW][16]; /* per (input state, moved) */
+	__shared__ uint32_t s_run[16], s_tc[16];
+	__shared__ uint32_t s_head[8], s_end[8]; /* the chunk's block of each input state */
 	__shared__ uint32_t s_acc[NW][8];
 	__shared__ uint32_t s_out[NFIELD][TILE]; /* updated agents of the tile */
 
@@ -802,9 +819,15 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				s_wlo = 0u;
 				s_wm = 0u;
 			}
-			if (tid < 8u) s_run[tid] = 0u;
+			if (tid < 16u) s_run[tid] = 0u;
 		}
 		__syncthreads();
+		if (tid < 8u) {
+			uint32_t h = T.ip.kstart[tid] > T.ch.k0 ? T.ip.kstart[tid] : T.ch.k0;
+			uint32_t z = T.ip.kstart[tid + 1u] < T.ch.k1 ? T.ip.kstart[tid + 1u] : T.ch.k1;
+			s_head[tid] = h;
+			s_end[tid] = z > h ? z : h;
+		}
 		const size_t P = T.P;
 		Acc acc = {0, 0, 0, 0, 0};
 		uint32_t l1de = 0, fem = 0, pinf = 0;
@@ -849,6 +872,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			}
 			__syncthreads();
 			const uint32_t m = s_wm, wlo = s_wlo;
+			/* the input states the tile spans (dense order is state-sorted) */
+			const uint32_t q_hi = seg_find(s_k0, 0u, m, te - 1u);
+			const uint32_t s_lo = s_st[seg_find(s_k0, 0u, m, ts)] & 0xffu;
+			const uint32_t s_hi = s_st[q_hi] & 0xffu;
 			uint32_t ns[APT], lr[APT];
 			/* gather: all of the tile's loads issued before any update */
 #pragma unroll
@@ -860,7 +887,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t r = k - s_k0[q];
 					Ag a;
 					if (s_pb[q] == PB_NEWBORN) newborn(a, T.nbseed, T.ip.nb_temp, s_vb[q] + r);
-					else load_agent(a, T.bin, P, s_pb[q] + r);
+					else load_agent(a, T.bin, P, seg_slot(s_pb[q], s_st[q], r));
 					s_out[F_AGE][o] = sc_bits_f(a.age);
 					s_out[F_HIS][o] = sc_bits_f(a.his);
 					s_out[F_DLY][o] = sc_bits_f(a.dly);
@@ -874,10 +901,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 #pragma unroll 1
 			for (int jx = 0; jx < APT; ++jx) {
 				const uint32_t k = ts + (uint32_t)jx * WG + tid;
-				uint32_t n2 = S_DEAD;
+				uint32_t n2 = 0xffu; /* class 2 * input state + moved, 0xff: dead or no agent */
 				if (k < te) {
 					const uint32_t q = seg_find(s_k0, 0u, m, k);
-					const uint32_t s = s_st[q];
+					const uint32_t s = s_st[q] & 0xffu;
 					const uint32_t g = s_vb[q] + (k - s_k0[q]);
 					const uint32_t o = (uint32_t)jx * WG + tid;
 					Ag a;
@@ -889,8 +916,9 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					a.avail = s_out[F_AVAIL][o];
 					a.cyc = s_out[F_CYC][o];
 					a.misc = s_out[F_MISC][o];
-					n2 = agent_step(a, s, g, T, acc);
-					if (n2 < S_DEAD) {
+					const uint32_t ns2 = agent_step(a, s, g, T, acc);
+					if (ns2 < S_DEAD) {
+						n2 = 2u * s + (ns2 != s ? 1u : 0u);
 						s_out[F_AGE][o] = sc_bits_f(a.age);
 						s_out[F_HIS][o] = sc_bits_f(a.his);
 						s_out[F_DLY][o] = sc_bits_f(a.dly);
@@ -899,44 +927,46 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 						s_out[F_AVAIL][o] = a.avail;
 						s_out[F_CYC][o] = a.cyc;
 						s_out[F_MISC][o] = a.misc;
-						if (n2 == S_LARVA) l1de += sc_f2u(a.age / 24.0f);
-						if (n2 >= S_IMM && (a.misc >> 31)) {
+						if (ns2 == S_LARVA) l1de += sc_f2u(a.age / 24.0f);
+						if (ns2 >= S_IMM && (a.misc >> 31)) {
 							fem += 1u;
 							pinf += a.csd >= 1.0f ? 1u : 0u;
 						}
 					}
 				}
-				/* rank survivors by new state */
+				/* rank survivors by (input state, moved) over the input states
+				 * the tile spans */
 				uint32_t rank = 0;
-#pragma unroll
-				for (uint32_t st = 0; st < 8u; ++st) {
-					uint64_t mk = __ballot(n2 == st);
-					if (n2 == st) rank = (uint32_t)__popcll(mk & lt);
-					if (lane == 0) s_wc[jx][w][st] = (uint32_t)__popcll(mk);
+				for (uint32_t cc = 2u * s_lo; cc <= 2u * s_hi + 1u; ++cc) {
+					uint64_t mk = __ballot(n2 == cc);
+					if (n2 == cc) rank = (uint32_t)__popcll(mk & lt);
+					if (lane == 0) s_wc[jx][w][cc] = (uint32_t)__popcll(mk);
 				}
 				ns[jx] = n2;
 				lr[jx] = rank;
 			}
 			__syncthreads();
-			if (tid < 8u) { /* exclusive over (jx, wave) per state, after the chunk's earlier tiles */
+			if (tid < 16u) { /* exclusive over (jx, wave) per class, after the chunk's earlier tiles */
 				uint32_t run = s_run[tid];
-				for (int jx = 0; jx < APT; ++jx)
-					for (int q = 0; q < NW; ++q) {
-						uint32_t cc = s_wc[jx][q][tid];
-						s_wc[jx][q][tid] = run;
-						run += cc;
-					}
+				if ((tid >> 1) >= s_lo && (tid >> 1) <= s_hi)
+					for (int jx = 0; jx < APT; ++jx)
+						for (int q = 0; q < NW; ++q) {
+							uint32_t cc = s_wc[jx][q][tid];
+							s_wc[jx][q][tid] = run;
+							run += cc;
+						}
 				s_tc[tid] = run;
 			}
 			__syncthreads();
 			/* the segment holding te - 1 starts the next tile's lookup */
-			next = wlo + seg_find(s_k0, 0u, m, te - 1u);
+			next = wlo + q_hi;
 #pragma unroll 1
 			for (int jx = 0; jx < APT; ++jx) {
-				const uint32_t st = ns[jx];
-				if (st < S_DEAD) {
+				const uint32_t cl = ns[jx];
+				if (cl < 16u) {
 					const uint32_t o = (uint32_t)jx * WG + tid;
-					const uint32_t p = T.ch.pout[st] + s_wc[jx][w][st] + lr[jx];
+					const uint32_t off = s_wc[jx][w][cl] + lr[jx];
+					const uint32_t p = (cl & 1u) ? s_end[cl >> 1] - 1u - off : s_head[cl >> 1] + off;
 					uint32_t *b = T.bout;
 					b[p] = s_out[F_AGE][o];
 					b[P + p] = s_out[F_HIS][o];
@@ -949,7 +979,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				}
 			}
 			__syncthreads();
-			if (tid < 8u) s_run[tid] = s_tc[tid];
+			if (tid < 16u) s_run[tid] = s_tc[tid];
 		}
 		{ /* chunk results */
 			uint32_t v[8] = {acc.eggs, acc.bites, acc.ibites, acc.ncyc, acc.scyc, l1de, fem, pinf};
@@ -962,9 +992,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			if (tid < 8u) {
 				uint32_t r = 0;
 				for (int q = 0; q < NW; ++q) r += s_acc[q][tid];
-				T.cout[(size_t)(8u + tid) * T.cmax + c] = r;
-				T.cout[(size_t)tid * T.cmax + c] = s_run[tid];
+				T.cout[(size_t)(CO_SUMS + tid) * T.cmax + c] = r;
 			}
+			if (tid < 16u) /* stayed: row q, moved on: row 8 + q */
+				T.cout[(size_t)((tid & 1u) * 8u + (tid >> 1)) * T.cmax + c] = s_run[tid];
 			if (tid == 0) insts[ii].pending = 1u;
 		}
 		__syncthreads();
@@ -991,7 +1022,7 @@ __global__ void k_export(const uint32_t *buf, uint32_t P, const Seg *seg, uint32
 	const Seg sg = seg[seg_find_global(seg, nseg, k)];
 	Ag a;
 	if (sg.pbase == PB_NEWBORN) newborn(a, nbz, nb_temp, sg.vbase + (k - sg.k0));
-	else load_agent(a, buf, P, sg.pbase + (k - sg.k0));
+	else load_agent(a, buf, P, seg_slot(sg.pbase, sg.state, k - sg.k0));
 	sampo_agent r;
 	r.ageInHours = a.age;
 	r.hoursInState = a.his;
@@ -1015,7 +1046,7 @@ __global__ void __launch_bounds__(256) k_histogram(const uint32_t *buf, const Se
 	__syncthreads();
 	for (uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
 		const Seg sg = seg[seg_find_global(seg, nseg, k)];
-		float age = sc_float(buf[sg.pbase + (k - sg.k0)]);
+		float age = sc_float(buf[seg_slot(sg.pbase, sg.state, k - sg.k0)]);
 		uint32_t b = sc_f2u(age / 24.0f);
 		atomicAdd(&bins[b < 99u ? b : 99u], 1u);
 	}
@@ -1200,7 +1231,9 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	if (sc->mode != SAMPO_MODE_REFERENCE && sc->mode != SAMPO_MODE_CONFIG)
 		return fail(SAMPO_E_ARG, "unknown mode%s%ld");
 	if (sc->max_agents == 0) return fail(SAMPO_E_ARG, "max_agents is required%s%ld");
-	uint64_t P64 = sc->phys_slots ? sc->phys_slots : 3ull * sc->max_agents + 1024ull;
+	/* an iteration writes exactly as many slots as it has agents, and a
+	 * running instance has at most 0.95 * max_agents */
+	uint64_t P64 = sc->phys_slots ? sc->phys_slots : sc->max_agents;
 	P64 = ((P64 + 255ull) / 256ull) * 256ull;
 	if (P64 > (1ull << 31)) return fail(SAMPO_E_ARG, "physical slots exceed 2^31%s%ld");
 	const uint32_t P = (uint32_t)P64;
@@ -1219,7 +1252,7 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	h.chunk_size = e->chunk;
 	h.target_chunks = e->target_chunks ? e->target_chunks : 2u * (uint32_t)e->grid;
 	h.cmax = (uint32_t)((P64 + e->chunk - 1u) / e->chunk) + 2u;
-	h.smax = 8u * h.cmax + 2u;
+	h.smax = 16u * h.cmax + 2u;
 	for (int c = 0; c < 4; ++c) h.hmr_stage[c] = sm_hmr(sc->dmr[c]);
 	for (uint32_t d = 0; d < 100; ++d) h.hmr_adult[d] = sc->mode == SAMPO_MODE_CONFIG ? h.hmr_stage[3] : sm_hmr_adult(d);
 	IterState &s0 = h.st[0];
