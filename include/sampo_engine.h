@@ -37,6 +37,7 @@ extern "C" {
 #define SAMPO_CAPACITY 2u      /* nAgents > 0.95 * maxAgents (src/abms.c:305) */
 #define SAMPO_EXTINCT 3u       /* nAgents == 0 (src/abms.c:307) */
 #define SAMPO_PHYS_FULL 4u     /* physical slots of the instance exhausted */
+#define SAMPO_GROW 5u          /* paused: its buffers need more slots; sampo_engine_service() grows them */
 #define SAMPO_EMPTY 0xffu      /* slot not in use */
 
 #define SAMPO_MODE_REFERENCE 0u /* killAgents.cl formulas, initial eggs via initAgents */
@@ -93,7 +94,8 @@ typedef struct {
 	uint32_t horizon;       /* iterations to run */
 	uint32_t n_eggs;        /* reference mode: initial eggs (src/abms.c:18) */
 	uint64_t max_agents;    /* maxAgents of the 95% capacity check (include/abms.h:16) */
-	uint64_t phys_slots;    /* physical agent slots per buffer; 0 = derived from max_agents */
+	uint64_t phys_slots;    /* initial agent slots per buffer (grown on demand up to max_agents
+	                           by sampo_engine_service); 0 = max_agents, never grown */
 	float dmr[4];           /* config mode: daily mortality egg, larva, pupa, adult */
 	const float *temp;      /* horizon temperatures, one per iteration */
 	const sampo_env *env;   /* horizon environments, or one if env_count == 1 */
@@ -128,6 +130,10 @@ int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chu
  * inputs and prepares iteration 0.  Returns the slot in *slot. */
 int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uint32_t *slot);
 
+/* Device bytes sampo_engine_admit() would allocate for this scenario with
+ * the given chunk size (0: the default 4096), for admission by footprint. */
+uint64_t sampo_engine_footprint(const sampo_scenario *sc, uint32_t chunk);
+
 /* Free a slot (its instance must not be running in an unsynchronised step). */
 int sampo_engine_release(sampo_engine *e, uint32_t slot);
 
@@ -151,6 +157,19 @@ int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_ms, float *p
 /* Device-to-device copy of `bytes` bytes with a plain 16-byte-per-lane
  * kernel, `iters` times; average milliseconds per copy. */
 int sampo_stream_copy(int device, uint64_t bytes, uint32_t iters, float *ms_per_copy);
+
+/* Status, current iteration and agents alive at its start for slots
+ * 0..n-1 in one read (synchronises); empty slots report SAMPO_EMPTY.  Any
+ * output pointer may be NULL. */
+int sampo_engine_poll(sampo_engine *e, uint32_t n, uint32_t *status, uint32_t *step, uint64_t *n_agents);
+
+/* Between steps (synchronises): give every instance that is paused in
+ * SAMPO_GROW, or running with fewer than headroom x its agents in physical
+ * slots, larger buffers (up to max_agents), and resume the paused ones.  A
+ * paused instance stops advancing until this is called, so its step lags
+ * the others'; keep stepping until every instance is done.  *grown receives
+ * the number of instances given new buffers. */
+int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *grown);
 
 /* Status, current iteration and agents alive at its start (synchronises). */
 int sampo_engine_status(sampo_engine *e, uint32_t slot, uint32_t *status, uint32_t *step,

@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ENGINE_LIB = os.path.join(HERE, "libsampo_engine.so")
 
-RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, EMPTY = 0, 1, 2, 3, 4, 0xFF
+RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, GROW, EMPTY = 0, 1, 2, 3, 4, 5, 0xFF
 MODE_REFERENCE, MODE_CONFIG = 0, 1
 STATES = ("egg", "larva", "pupa", "immature", "mateseeking", "bms", "bmd", "gravid")
 
@@ -85,6 +85,10 @@ def lib():
         L.sampo_engine_set_chunk.argtypes = [vp, C.c_uint32, C.c_uint32]
         L.sampo_engine_admit.argtypes = [vp, C.POINTER(Scenario), C.POINTER(C.c_uint32)]
         L.sampo_engine_release.argtypes = [vp, C.c_uint32]
+        L.sampo_engine_footprint.argtypes = [C.POINTER(Scenario), C.c_uint32]
+        L.sampo_engine_footprint.restype = C.c_uint64
+        L.sampo_engine_poll.argtypes = [vp, C.c_uint32, vp, vp, vp]
+        L.sampo_engine_service.argtypes = [vp, C.c_float, C.POINTER(C.c_uint32)]
         L.sampo_engine_step.argtypes = [vp, C.c_uint32]
         L.sampo_engine_step_days.argtypes = [vp, C.c_uint32]
         L.sampo_engine_sync.argtypes = [vp]
@@ -143,6 +147,38 @@ def make_env(bloodmeal=0.25, itn=0.0, irs=0.0, cc=160000.0, larvicide=0.0, ovitr
                     dtype=[(n, "<f4") for n, _ in Env._fields_])
 
 
+def scenario(*, mode=MODE_REFERENCE, seed=1, horizon, temp, env, n_eggs=0, max_agents, phys_slots=0,
+             dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):
+    """A sampo_scenario and the arrays it points into (keep them alive)."""
+    temp = np.ascontiguousarray(temp, dtype=np.float32)
+    env = np.ascontiguousarray(env)
+    if temp.shape[0] < horizon:
+        raise ValueError("temp shorter than horizon")
+    sc = Scenario()
+    sc.mode, sc.seed, sc.horizon, sc.n_eggs = mode, seed, horizon, n_eggs
+    sc.max_agents, sc.phys_slots = max_agents, phys_slots
+    sc.dmr = (C.c_float * 4)(*dmr)
+    sc.temp = temp.ctypes.data_as(C.POINTER(C.c_float))
+    sc.env = C.cast(env.ctypes.data, C.POINTER(Env))
+    sc.env_count = env.shape[0]
+    keep = [temp, env]
+    if layout is not None:
+        lay = Layout()
+        for s in range(8):
+            lay.start[s], lay.end[s] = int(layout[s][0]), int(layout[s][1])
+        agents = np.ascontiguousarray(agents, dtype=AGENT_DTYPE)
+        sc.init_layout = C.pointer(lay)
+        sc.init_agents = C.cast(agents.ctypes.data, C.POINTER(Agent))
+        keep += [lay, agents]
+    return sc, keep
+
+
+def footprint(chunk=0, **kw):
+    """Device bytes admitting this scenario would allocate."""
+    sc, keep = scenario(**kw)
+    return int(lib().sampo_engine_footprint(C.byref(sc), chunk))
+
+
 class Engine:
     """A set of scenario instances advanced together on one GPU."""
 
@@ -179,33 +215,30 @@ class Engine:
         admissions."""
         _check(lib().sampo_engine_set_chunk(self.h, agents, target_chunks))
 
-    def admit(self, *, mode=MODE_REFERENCE, seed=1, horizon, temp, env, n_eggs=0, max_agents,
-              phys_slots=0, dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):
-        temp = np.ascontiguousarray(temp, dtype=np.float32)
-        env = np.ascontiguousarray(env)
-        if temp.shape[0] < horizon:
-            raise ValueError("temp shorter than horizon")
-        sc = Scenario()
-        sc.mode, sc.seed, sc.horizon, sc.n_eggs = mode, seed, horizon, n_eggs
-        sc.max_agents, sc.phys_slots = max_agents, phys_slots
-        sc.dmr = (C.c_float * 4)(*dmr)
-        sc.temp = temp.ctypes.data_as(C.POINTER(C.c_float))
-        sc.env = C.cast(env.ctypes.data, C.POINTER(Env))
-        sc.env_count = env.shape[0]
-        lay = None
-        if layout is not None:
-            lay = Layout()
-            for s in range(8):
-                lay.start[s], lay.end[s] = int(layout[s][0]), int(layout[s][1])
-            agents = np.ascontiguousarray(agents, dtype=AGENT_DTYPE)
-            sc.init_layout = C.pointer(lay)
-            sc.init_agents = C.cast(agents.ctypes.data, C.POINTER(Agent))
+    def admit(self, **kw):
+        """Admit a scenario (keywords of scenario()); returns its slot."""
+        sc, keep = scenario(**kw)
         slot = C.c_uint32()
         _check(lib().sampo_engine_admit(self.h, C.byref(sc), C.byref(slot)))
+        del keep
         return slot.value
 
     def release(self, slot):
         _check(lib().sampo_engine_release(self.h, slot))
+
+    def poll(self, n):
+        """(status, step, agents) arrays for slots 0..n-1 in one read."""
+        st = np.zeros(n, np.uint32)
+        step = np.zeros(n, np.uint32)
+        na = np.zeros(n, np.uint64)
+        _check(lib().sampo_engine_poll(self.h, n, st.ctypes.data, step.ctypes.data, na.ctypes.data))
+        return st, step, na
+
+    def service(self, headroom=1.25):
+        """Grow instances paused for slots or short of headroom; returns how many."""
+        g = C.c_uint32()
+        _check(lib().sampo_engine_service(self.h, headroom, C.byref(g)))
+        return g.value
 
     def step(self, n):
         _check(lib().sampo_engine_step(self.h, n))

@@ -110,6 +110,7 @@ struct Inst {
 	sampo_row *rows;       /* horizon + 1 */
 	uint64_t max_agents;
 	uint32_t P, horizon, mode, env_count;
+	uint32_t P_max;        /* physical slots the buffers may grow to */
 	uint32_t chunk_size, target_chunks, cmax, smax;
 	uint32_t status, step, pending, needs_begin;
 	uint32_t l1de, females, potinf, ncyc, scyc, bites, ibites; /* stats of the start of `step` */
@@ -641,8 +642,8 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		} else if (n == 0u) {
 			st = SAMPO_EXTINCT;
 			row.valid = 0u;
-		} else if (n > I->P) {
-			st = SAMPO_PHYS_FULL; /* before cutting chunks: at most cmax of them exist */
+		} else if (n > I->P) { /* the output would not fit: grow, or give up */
+			st = I->P < I->P_max ? SAMPO_GROW : SAMPO_PHYS_FULL;
 		} else {
 			st = SAMPO_RUNNING;
 		}
@@ -1055,6 +1056,51 @@ __global__ void __launch_bounds__(256) k_histogram(const uint32_t *buf, const Se
 		if (bins[q]) atomicAdd(&hist[q], bins[q]);
 }
 
+/* Initial population: host records (dense, range order) into the SoA
+ * buffer, and the calcStats sums of the start (kernel/calcL1de.cl,
+ * kernel/calcGender.cl) into the instance. */
+__global__ void __launch_bounds__(256) k_import(const sampo_agent *in, uint32_t n, uint32_t k_larva0,
+                                                uint32_t k_larva1, uint32_t k_adult0, uint32_t *buf, uint32_t P,
+                                                Inst *inst)
+{
+	__shared__ uint32_t s_sum[3][4];
+	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t l1de = 0, fem = 0, pinf = 0;
+	if (k < n) {
+		const sampo_agent a = in[k];
+		const size_t Pz = P;
+		buf[k] = sc_bits_f(a.ageInHours);
+		buf[Pz + k] = sc_bits_f(a.hoursInState);
+		buf[2 * Pz + k] = sc_bits_f(a.delay);
+		buf[3 * Pz + k] = a.aux;
+		buf[4 * Pz + k] = sc_bits_f(a.csd);
+		buf[5 * Pz + k] = a.availableEggs;
+		buf[6 * Pz + k] = a.cycleLength;
+		buf[7 * Pz + k] = (a.humanBloodmealCount & 0xffffu) | ((a.numEggBatches & 0x7fffu) << 16) |
+		                  (a.isFemale ? 0x80000000u : 0u);
+		if (k >= k_larva0 && k < k_larva1) l1de = sc_f2u(a.ageInHours / 24.0f);
+		if (k >= k_adult0 && a.isFemale) {
+			fem = 1u;
+			pinf = a.csd >= 1.0f ? 1u : 0u;
+		}
+	}
+	l1de = wave_sum(l1de);
+	fem = wave_sum(fem);
+	pinf = wave_sum(pinf);
+	const uint32_t w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63u) == 0) {
+		s_sum[0][w] = l1de;
+		s_sum[1][w] = fem;
+		s_sum[2][w] = pinf;
+	}
+	__syncthreads();
+	if (threadIdx.x < 3u) {
+		uint32_t r = s_sum[threadIdx.x][0] + s_sum[threadIdx.x][1] + s_sum[threadIdx.x][2] + s_sum[threadIdx.x][3];
+		uint32_t *dst = threadIdx.x == 0 ? &inst->l1de : threadIdx.x == 1 ? &inst->females : &inst->potinf;
+		if (r) atomicAdd(dst, r);
+	}
+}
+
 /* ============================================================ host side */
 
 namespace {
@@ -1076,7 +1122,8 @@ int fail(int code, const char *fmt, const char *a = "", long b = 0)
 
 struct Slot {
 	bool used = false;
-	void *mem = nullptr; /* one allocation: buffers, segments, chunks, inputs */
+	void *mem = nullptr;    /* segments, chunks, counters, inputs, rows */
+	void *agents = nullptr; /* the two SoA agent buffers, 2 * NFIELD * P dwords */
 	uint32_t horizon = 0;
 	uint32_t P = 0;
 };
@@ -1181,6 +1228,7 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 static void free_slot(Slot &s)
 {
 	if (s.mem) (void)hipFree(s.mem);
+	if (s.agents) (void)hipFree(s.agents);
 	s = Slot();
 }
 
@@ -1222,6 +1270,56 @@ extern "C" int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t
 	return 0;
 }
 
+/* layout of one instance's bookkeeping allocation (all but the agent buffers) */
+struct SlotLayout {
+	size_t o_seg0, o_seg1, o_ch0, o_ch1, o_cout, o_seeds, o_temp, o_env, o_rows, total;
+	uint32_t cmax, smax;
+};
+
+static SlotLayout slot_layout(uint64_t P_max, uint32_t chunk, uint32_t horizon, uint32_t env_count)
+{
+	SlotLayout L;
+	L.cmax = (uint32_t)((P_max + chunk - 1u) / chunk) + 2u;
+	L.smax = 16u * L.cmax + 2u;
+	size_t off = 0;
+	L.o_seg0 = off; off = align256(off + (size_t)L.smax * sizeof(Seg));
+	L.o_seg1 = off; off = align256(off + (size_t)L.smax * sizeof(Seg));
+	L.o_ch0 = off; off = align256(off + (size_t)L.cmax * sizeof(Chunk));
+	L.o_ch1 = off; off = align256(off + (size_t)L.cmax * sizeof(Chunk));
+	L.o_cout = off; off = align256(off + (size_t)NOUT * L.cmax * sizeof(uint32_t));
+	L.o_seeds = off; off = align256(off + (size_t)(horizon + 1u) * 16u * sizeof(uint32_t));
+	L.o_temp = off; off = align256(off + (size_t)horizon * sizeof(float));
+	L.o_env = off; off = align256(off + (size_t)env_count * sizeof(sampo_env));
+	L.o_rows = off; off = align256(off + (size_t)(horizon + 1u) * sizeof(sampo_row));
+	L.total = off;
+	return L;
+}
+
+static uint64_t round256(uint64_t v) { return ((v + 255ull) / 256ull) * 256ull; }
+
+/* physical slots: what the caller asked for, else max_agents; never fewer
+ * than the initial population, never more than max_agents */
+static uint64_t initial_slots(const sampo_scenario *sc, uint64_t n0)
+{
+	uint64_t pmax = round256(sc->max_agents);
+	uint64_t P = sc->phys_slots ? round256(sc->phys_slots) : pmax;
+	if (P < round256(n0)) P = round256(n0);
+	if (P < 256u) P = 256u;
+	return P < pmax ? P : pmax;
+}
+
+extern "C" uint64_t sampo_engine_footprint(const sampo_scenario *sc, uint32_t chunk)
+{
+	if (!sc || sc->horizon == 0 || sc->max_agents == 0) return 0;
+	uint64_t n0 = sc->mode == SAMPO_MODE_REFERENCE ? sc->n_eggs : 0u;
+	if (sc->mode == SAMPO_MODE_CONFIG && sc->init_layout)
+		for (int s = 0; s < 8; ++s) n0 += sc->init_layout->end[s] - sc->init_layout->start[s];
+	const uint64_t P = initial_slots(sc, n0);
+	const SlotLayout L = slot_layout(round256(sc->max_agents), chunk ? chunk : 4096u, sc->horizon,
+	                                 sc->env_count ? sc->env_count : 1u);
+	return (uint64_t)L.total + 2ull * NFIELD * 4ull * P;
+}
+
 extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uint32_t *slot_out)
 {
 	if (!e || !sc || !slot_out) return fail(SAMPO_E_ARG, "null argument%s%ld");
@@ -1231,12 +1329,7 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	if (sc->mode != SAMPO_MODE_REFERENCE && sc->mode != SAMPO_MODE_CONFIG)
 		return fail(SAMPO_E_ARG, "unknown mode%s%ld");
 	if (sc->max_agents == 0) return fail(SAMPO_E_ARG, "max_agents is required%s%ld");
-	/* an iteration writes exactly as many slots as it has agents, and a
-	 * running instance has at most 0.95 * max_agents */
-	uint64_t P64 = sc->phys_slots ? sc->phys_slots : sc->max_agents;
-	P64 = ((P64 + 255ull) / 256ull) * 256ull;
-	if (P64 > (1ull << 31)) return fail(SAMPO_E_ARG, "physical slots exceed 2^31%s%ld");
-	const uint32_t P = (uint32_t)P64;
+	if (round256(sc->max_agents) > (1ull << 31)) return fail(SAMPO_E_ARG, "max_agents exceeds 2^31%s%ld");
 	uint32_t id = e->max_slots;
 	for (uint32_t i = 0; i < e->max_slots; ++i)
 		if (!e->slots[i].used) { id = i; break; }
@@ -1244,23 +1337,20 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 
 	Inst h;
 	memset(&h, 0, sizeof(h));
-	h.P = P;
 	h.horizon = sc->horizon;
 	h.mode = sc->mode;
 	h.env_count = sc->env_count;
 	h.max_agents = sc->max_agents;
 	h.chunk_size = e->chunk;
 	h.target_chunks = e->target_chunks ? e->target_chunks : 2u * (uint32_t)e->grid;
-	h.cmax = (uint32_t)((P64 + e->chunk - 1u) / e->chunk) + 2u;
-	h.smax = 16u * h.cmax + 2u;
+	h.P_max = (uint32_t)round256(sc->max_agents);
 	for (int c = 0; c < 4; ++c) h.hmr_stage[c] = sm_hmr(sc->dmr[c]);
 	for (uint32_t d = 0; d < 100; ++d) h.hmr_adult[d] = sc->mode == SAMPO_MODE_CONFIG ? h.hmr_stage[3] : sm_hmr_adult(d);
 	IterState &s0 = h.st[0];
 
 	/* initial population (src/abms.c:185-228, or the given one) and its segments */
 	std::vector<Seg> segs;
-	std::vector<uint32_t> soa;
-	size_t stored = 0;
+	uint64_t stored = 0;
 	if (sc->mode == SAMPO_MODE_REFERENCE) {
 		if ((uint64_t)sc->n_eggs > sc->max_agents) return fail(SAMPO_E_ARG, "n_eggs exceeds max_agents%s%ld");
 		s0.cnt[0] = sc->n_eggs;
@@ -1280,82 +1370,69 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 			if (s0.cnt[s]) segs.push_back(Seg{(uint32_t)k, (uint32_t)s, (uint32_t)k, L->start[s]});
 			k += s0.cnt[s];
 		}
-		stored = (size_t)k;
-		if (stored > P) return fail(SAMPO_E_ARG, "initial population exceeds physical slots%s%ld");
-		soa.assign((size_t)NFIELD * stored, 0u);
-		k = 0;
-		for (int s = 0; s < 8; ++s) {
-			for (uint32_t r = 0; r < s0.cnt[s]; ++r, ++k) {
-				const sampo_agent &a = sc->init_agents[k];
-				soa[F_AGE * stored + k] = sc_bits_f(a.ageInHours);
-				soa[F_HIS * stored + k] = sc_bits_f(a.hoursInState);
-				soa[F_DLY * stored + k] = sc_bits_f(a.delay);
-				soa[F_AUX * stored + k] = a.aux;
-				soa[F_CSD * stored + k] = sc_bits_f(a.csd);
-				soa[F_AVAIL * stored + k] = a.availableEggs;
-				soa[F_CYC * stored + k] = a.cycleLength;
-				soa[F_MISC * stored + k] = (a.humanBloodmealCount & 0xffffu) | ((a.numEggBatches & 0x7fffu) << 16) |
-				                           (a.isFemale ? 0x80000000u : 0u);
-				/* calcStats reductions of the initial population (calcL1de.cl, calcGender.cl) */
-				if (s == S_LARVA) h.l1de += sc_f2u(a.ageInHours / 24.0f);
-				if (s >= S_IMM && a.isFemale) {
-					h.females += 1u;
-					h.potinf += a.csd >= 1.0f ? 1u : 0u;
-				}
-			}
-		}
+		stored = k;
+		if (stored > sc->max_agents) return fail(SAMPO_E_ARG, "initial population exceeds max_agents%s%ld");
 	}
+	const uint64_t P64 = initial_slots(sc, sc->mode == SAMPO_MODE_REFERENCE ? (uint64_t)sc->n_eggs : stored);
+	const uint32_t P = (uint32_t)P64;
+	h.P = P;
 	s0.nseg = (uint32_t)segs.size();
 	s0.step = 0;
 	s0.cur = 0;
 	h.step = 0;
 	h.status = SAMPO_RUNNING;
 	h.needs_begin = 1u;
-
-	/* one device allocation per instance */
-	const size_t seeds_n = (size_t)(sc->horizon + 1u) * 16u;
-	size_t off = 0;
-	const size_t o_buf = off; off = align256(off + (size_t)2 * NFIELD * P * sizeof(uint32_t));
-	const size_t o_seg0 = off; off = align256(off + (size_t)h.smax * sizeof(Seg));
-	const size_t o_seg1 = off; off = align256(off + (size_t)h.smax * sizeof(Seg));
-	const size_t o_ch0 = off; off = align256(off + (size_t)h.cmax * sizeof(Chunk));
-	const size_t o_ch1 = off; off = align256(off + (size_t)h.cmax * sizeof(Chunk));
-	const size_t o_cout = off; off = align256(off + (size_t)NOUT * h.cmax * sizeof(uint32_t));
-	const size_t o_seeds = off; off = align256(off + seeds_n * sizeof(uint32_t));
-	const size_t o_temp = off; off = align256(off + (size_t)sc->horizon * sizeof(float));
-	const size_t o_env = off; off = align256(off + (size_t)sc->env_count * sizeof(sampo_env));
-	const size_t o_rows = off; off = align256(off + (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
+	const SlotLayout L = slot_layout(h.P_max, e->chunk, sc->horizon, sc->env_count);
+	h.cmax = L.cmax;
+	h.smax = L.smax;
 
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
 	Slot S;
 	S.horizon = sc->horizon;
 	S.P = P;
-	hipError_t err = hipMalloc(&S.mem, off);
-	if (err != hipSuccess) return fail(SAMPO_E_NOMEM, "device allocation failed: %s%ld", hipGetErrorString(err));
+	hipError_t err = hipMalloc(&S.mem, L.total);
+	if (err == hipSuccess) err = hipMalloc(&S.agents, (size_t)2 * NFIELD * P * sizeof(uint32_t));
+	if (err != hipSuccess) {
+		(void)hipGetLastError();
+		free_slot(S);
+		return fail(SAMPO_E_NOMEM, "device allocation failed: %s%ld", hipGetErrorString(err));
+	}
 	char *base = (char *)S.mem;
+	const size_t seeds_n = (size_t)(sc->horizon + 1u) * 16u;
 	std::vector<uint32_t> seeds(seeds_n);
 	sampo_seed_stream(sc->seed, (uint32_t)seeds_n, seeds.data());
-	h.buf[0] = (uint32_t *)(base + o_buf);
+	h.buf[0] = (uint32_t *)S.agents;
 	h.buf[1] = h.buf[0] + (size_t)NFIELD * P;
-	h.seg[0] = (Seg *)(base + o_seg0);
-	h.seg[1] = (Seg *)(base + o_seg1);
-	h.chunk[0] = (Chunk *)(base + o_ch0);
-	h.chunk[1] = (Chunk *)(base + o_ch1);
-	h.cout = (uint32_t *)(base + o_cout);
-	h.seeds = (const uint32_t *)(base + o_seeds);
-	h.temp = (const float *)(base + o_temp);
-	h.env = (const sampo_env *)(base + o_env);
-	h.rows = (sampo_row *)(base + o_rows);
+	h.seg[0] = (Seg *)(base + L.o_seg0);
+	h.seg[1] = (Seg *)(base + L.o_seg1);
+	h.chunk[0] = (Chunk *)(base + L.o_ch0);
+	h.chunk[1] = (Chunk *)(base + L.o_ch1);
+	h.cout = (uint32_t *)(base + L.o_cout);
+	h.seeds = (const uint32_t *)(base + L.o_seeds);
+	h.temp = (const float *)(base + L.o_temp);
+	h.env = (const sampo_env *)(base + L.o_env);
+	h.rows = (sampo_row *)(base + L.o_rows);
 	err = hipMemcpy((void *)h.seeds, seeds.data(), seeds_n * sizeof(uint32_t), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy((void *)h.temp, sc->temp, (size_t)sc->horizon * sizeof(float), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy((void *)h.env, sc->env, (size_t)sc->env_count * sizeof(sampo_env), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemset(h.rows, 0, (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
 	if (err == hipSuccess && !segs.empty()) err = hipMemcpy(h.seg[0], segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice);
-	for (int f = 0; f < NFIELD && err == hipSuccess && stored; ++f)
-		err = hipMemcpy(h.buf[0] + (size_t)f * P, soa.data() + (size_t)f * stored, stored * sizeof(uint32_t),
-		                hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy(e->d_inst + id, &h, sizeof(Inst), hipMemcpyHostToDevice);
+	if (err == hipSuccess && stored) { /* host records -> SoA, on the device */
+		sampo_agent *d_in = nullptr;
+		err = hipMalloc(&d_in, (size_t)stored * sizeof(sampo_agent));
+		if (err == hipSuccess) err = hipMemcpy(d_in, sc->init_agents, (size_t)stored * sizeof(sampo_agent), hipMemcpyHostToDevice);
+		if (err == hipSuccess) {
+			const uint32_t k_l0 = s0.cnt[0], k_l1 = k_l0 + s0.cnt[1];
+			const uint32_t k_a0 = k_l1 + s0.cnt[2];
+			hipLaunchKernelGGL(k_import, dim3((uint32_t)((stored + 255u) / 256u)), dim3(256), 0, e->stream, d_in,
+			                   (uint32_t)stored, k_l0, k_l1, k_a0, h.buf[0], P, e->d_inst + id);
+			err = hipGetLastError();
+			if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+		}
+		if (d_in) (void)hipFree(d_in);
+	}
 	if (err != hipSuccess) {
 		free_slot(S);
 		return fail(SAMPO_E_HIP, "upload failed: %s%ld", hipGetErrorString(err));
@@ -1379,6 +1456,86 @@ extern "C" int sampo_engine_release(sampo_engine *e, uint32_t slot)
 	HIPCHK(hipMemcpy(&e->d_inst[slot].status, &st, sizeof(st), hipMemcpyHostToDevice));
 	free_slot(e->slots[slot]);
 	return 0;
+}
+
+extern "C" int sampo_engine_poll(sampo_engine *e, uint32_t n, uint32_t *status, uint32_t *step, uint64_t *n_agents)
+{
+	if (!e || n > e->max_slots) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	std::vector<Inst> h(n);
+	if (n) HIPCHK(hipMemcpy(h.data(), e->d_inst, sizeof(Inst) * n, hipMemcpyDeviceToHost));
+	for (uint32_t i = 0; i < n; ++i) {
+		const bool used = e->slots[i].used;
+		if (status) status[i] = used ? h[i].status : SAMPO_EMPTY;
+		if (step) step[i] = used ? h[i].step : 0u;
+		if (n_agents) n_agents[i] = used ? h[i].st[h[i].step & 1u].n : 0u;
+	}
+	return 0;
+}
+
+/* Move one instance's agents into buffers of P2 slots per field. */
+static int grow_slot(sampo_engine *e, uint32_t id, Inst &h, uint32_t P2)
+{
+	Slot &S = e->slots[id];
+	void *mem = nullptr;
+	hipError_t err = hipMalloc(&mem, (size_t)2 * NFIELD * P2 * sizeof(uint32_t));
+	if (err != hipSuccess) {
+		(void)hipGetLastError();
+		return fail(SAMPO_E_NOMEM, "growing an instance failed: %s%ld", hipGetErrorString(err));
+	}
+	uint32_t *nb = (uint32_t *)mem;
+	const uint32_t cur = h.st[h.step & 1u].cur;
+	/* the current buffer holds the live agents in its first P slots per field */
+	for (int f = 0; f < NFIELD; ++f)
+		HIPCHK(hipMemcpyAsync(nb + ((size_t)cur * NFIELD + f) * P2, h.buf[cur] + (size_t)f * h.P,
+		                      (size_t)h.P * sizeof(uint32_t), hipMemcpyDeviceToDevice, e->stream));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	(void)hipFree(S.agents);
+	S.agents = mem;
+	S.P = P2;
+	h.buf[0] = nb;
+	h.buf[1] = nb + (size_t)NFIELD * P2;
+	h.P = P2;
+	return 0;
+}
+
+extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *grown)
+{
+	if (!e || !(headroom >= 1.0f)) return fail(SAMPO_E_ARG, "headroom must be >= 1%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	std::vector<Inst> h(e->max_slots);
+	HIPCHK(hipMemcpy(h.data(), e->d_inst, sizeof(Inst) * e->max_slots, hipMemcpyDeviceToHost));
+	uint32_t count = 0, resumed = 0;
+	int rc = 0;
+	for (uint32_t i = 0; i < e->max_slots && !rc; ++i) {
+		if (!e->slots[i].used) continue;
+		Inst &I = h[i];
+		const uint64_t n = I.st[I.step & 1u].n;
+		const uint64_t want = (uint64_t)((double)n * headroom) + 1u;
+		const bool paused = I.status == SAMPO_GROW;
+		if (!paused && !(I.status == SAMPO_RUNNING && want > I.P && I.P < I.P_max)) continue;
+		uint64_t P2 = round256(want + want / 4u);
+		if (P2 < round256((uint64_t)I.P + I.P / 4u)) P2 = round256((uint64_t)I.P + I.P / 4u);
+		if (P2 > I.P_max) P2 = I.P_max;
+		rc = grow_slot(e, i, I, (uint32_t)P2);
+		if (rc) break;
+		if (paused) { /* run its calcStats again and continue */
+			I.status = SAMPO_RUNNING;
+			I.needs_begin = 1u;
+			++resumed;
+		}
+		HIPCHK(hipMemcpy(e->d_inst + i, &I, sizeof(Inst), hipMemcpyHostToDevice));
+		++count;
+	}
+	if (resumed) {
+		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst);
+		HIPCHK(hipGetLastError());
+		HIPCHK(hipStreamSynchronize(e->stream));
+	}
+	if (grown) *grown = count;
+	return rc;
 }
 
 static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after_pass)

@@ -119,6 +119,65 @@ def test_engine_batched_instances_equal_solo_runs():
             agents_equal(a, e.export(s)[1])
 
 
+@pytest.mark.parametrize("chunk", [64, 200, 1000])
+def test_engine_small_chunks_match_oracle(chunk):
+    """Many chunks per state: blocks split across chunks, tiles that span
+    states, runs moved in from the previous state in every chunk."""
+    n0, days, seed, dmr = 6000, 6, 31, (0.05, 0.03, 0.02, 0.1)
+    steps = 24 * days
+    temp, lay, ag = config_inputs(n0, days, seed)
+    env = S.make_env(cc=0.05 * n0)
+    sim = O.Sim(temp=temp, env=env, capacity=60 * n0, seed=seed, layout=lay, agents=ag, mode=1, dmr=dmr)
+    orows = sim.run(steps)
+    with S.Engine(max_slots=1) as e:
+        e.set_chunk(chunk, 1 << 20)
+        slot = e.admit(mode=S.MODE_CONFIG, seed=seed, horizon=steps, temp=temp, env=env, max_agents=60 * n0,
+                       dmr=dmr, layout=lay, agents=ag)
+        e.step(steps)
+        rows_equal(orows, e.rows(slot, 0, steps))
+        lay_o, ag_o, _ = sim.export()
+        lay_g, ag_g = e.export(slot)
+        assert np.array_equal(lay_o, lay_g)
+        agents_equal(ag_o, ag_g)
+        assert np.array_equal(sim.histogram(), e.histogram(slot))
+
+
+def test_engine_growth_is_transparent():
+    """Instances admitted with only as many slots as their first population
+    pause when the output would not fit, grow between days and resume; each
+    result is still the oracle's, bit for bit."""
+    days = 30  # both populations fall for ~11 days, then outgrow their first buffers
+    steps = 24 * days
+    specs, sims = [], []
+    for i, n0 in enumerate((4000, 700)):
+        temp, lay, ag = config_inputs(n0, days, 40 + i)
+        env = S.make_env(cc=0.025 * n0)
+        dmr = (0.05, 0.03, 0.02, 0.1)
+        sim = O.Sim(temp=temp, env=env, capacity=40 * n0, seed=40 + i, layout=lay, agents=ag, mode=1, dmr=dmr)
+        sims.append((sim.run(steps), sim.export()))
+        specs.append(dict(mode=S.MODE_CONFIG, seed=40 + i, horizon=steps, temp=temp, env=env, max_agents=40 * n0,
+                          phys_slots=n0, dmr=dmr, layout=lay, agents=ag))
+    with S.Engine(max_slots=4) as e:
+        slots = [e.admit(**sp) for sp in specs]
+        grown, paused_seen, rounds = 0, False, 0
+        while True:
+            st, step, _ = e.poll(4)
+            if all(st[s] == S.DONE for s in slots):
+                break
+            assert all(st[s] in (S.RUNNING, S.GROW) for s in slots), st
+            paused_seen |= any(st[s] == S.GROW for s in slots)
+            grown += e.service(1.0)  # grow only when an instance must
+            e.step_days(1)
+            rounds += 1
+            assert rounds < 4 * days
+        assert grown > 0 and paused_seen
+        for s, (orows, (lay_o, ag_o, _)) in zip(slots, sims):
+            rows_equal(orows, e.rows(s, 0, steps))
+            lay_g, ag_g = e.export(s)
+            assert np.array_equal(lay_o, lay_g)
+            agents_equal(ag_o, ag_g)
+
+
 def test_engine_tile_shapes_agree():
     steps = 24 * 4
     temp, env = ref_inputs(steps, 20000.0)
