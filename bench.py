@@ -14,7 +14,17 @@ Default workload: plain single run, large -- 32,000,000 initial agents (age
 stage mortality 0.05/0.03/0.02/0.10 per day, one fused HIP pass per
 iteration.
 
-Besides the JSON line's throughput, the line carries:
+Batched workloads (--workload ensemble | sweep): every scenario of the set
+is an instance of one engine, advanced by one launch of each kernel per
+iteration and one HIP graph per day; value = agent-steps/s of the whole set
+over the timed days.  The same line reports the plain paths on a uniform
+sample of the set, one scenario at a time -- the fused engine alone, and the
+reference's driver loop over the icl boundary -- and checks that each
+sampled scenario's printStats rows are identical in the batched run, alone,
+and through the driver loop.  Launch parameters come from the autotuner in
+sampo_amd/batch.py (per initial-size bucket), run before the timed region.
+
+Besides the JSON line's throughput, the single-run line carries:
   roofline     algorithmic bytes (34 B per agent in + 34 B per agent out) per
                fused-pass launch / that launch's duration from HIP events
                recorded around every pass of the timed region;
@@ -90,12 +100,110 @@ def cpu_baseline(w, days, scale):
                       f"{steps} agent-steps in {dt:.1f} s"}
 
 
+def batched_main(args):
+    import sampo_amd as S
+    from sampo_amd import batch as B
+    from sampo_amd import workloads as W
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if args.workload == "ensemble":
+        specs = W.ensemble(count=args.instances or 1024)
+        desc = ("batched ensemble: replicates of the single-run parameters with distinct seeds, initial "
+                "populations uniform 1e5-1e6, 365-day horizon, CC = 0.025 x N0")
+    else:
+        specs = W.sweep(count=args.instances or 4096)
+        desc = ("uneven parameter sweep: initial population log-uniform 1e3-4e6, mean temperature 18-32 degC, "
+                "amplitude 2-8 degC, per-stage mortality multipliers 0.5-1.5, 365-day horizon, CC = 0.0125 x N0")
+    n_total = sum(sp.n0 for sp in specs)
+    log(f"{args.workload}: {len(specs)} scenarios, {n_total / 1e9:.3f}e9 initial agents")
+    t0 = time.perf_counter()
+    if args.no_tune:
+        tun = B.Tuning()
+    else:
+        tun = B.autotune(specs, log=log)
+    tune_s = time.perf_counter() - t0
+    log(f"tuning {tune_s:.1f} s: tile {tun.tile}, chunk per bucket {tun.chunk}")
+    sample = sorted(set(np.linspace(0, len(specs) - 1, min(args.plain_sample, len(specs))).round().astype(int)))
+    bres, brows = B.run_batched(specs, tun, warm_days=args.warmup, timed_days=args.steps, keep_rows=sample, log=log)
+    log(f"batched: {bres['agent_steps_per_s']:.3e} agent-steps/s, day {bres['day_ms']:.1f} ms")
+    pres, prows = B.run_plain_engine([specs[i] for i in sample], days=1, warm_days=1)
+    log(f"plain engine on {len(sample)} scenarios: {pres['agent_steps_per_s']:.3e} agent-steps/s")
+    isample = sample[::max(1, len(sample) // max(1, args.icl_sample))][:args.icl_sample]
+    ires, irows = B.run_plain_icl([specs[i] for i in isample], days=1)
+    log(f"plain icl driver on {len(isample)} scenarios: {ires['agent_steps_per_s']:.3e} agent-steps/s")
+    n_cmp = 24 * (args.warmup + 1) + 1
+    ident = sum(B.same_rows(brows[i], r, n_cmp) for i, r in zip(sample, prows))
+    ident_icl = sum(B.same_rows(brows[i], r, 24) for i, r in zip(isample, irows))
+    out = {
+        "metric": METRIC, "value": bres["agent_steps_per_s"], "unit": "agent-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": bres["day_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
+        "config": {"workload": desc, "instances": len(specs), "initial_agents": n_total, "timed_days": args.steps,
+                   "iterations_per_day": 24, "tile": "x".join(map(str, tun.tile)),
+                   "chunk_per_bucket": {str(k): v for k, v in sorted(tun.chunk.items())},
+                   "parallelism": "1 GPU"},
+        "agent_days_per_s": bres["agent_steps_per_s"] / 24.0,
+        "batched": bres, "plain_engine": pres, "plain_icl": ires,
+        "speedup_vs_plain_engine": bres["agent_steps_per_s"] / pres["agent_steps_per_s"],
+        "speedup_vs_plain_icl": bres["agent_steps_per_s"] / ires["agent_steps_per_s"],
+        "identity": {"checked": len(sample), "identical_to_solo": int(ident), "rows_compared": n_cmp,
+                     "checked_icl": len(isample), "identical_to_icl_driver": int(ident_icl), "rows_compared_icl": 24},
+        "roofline": {"bound": "hbm", "achieved": bres["pass_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bres["pass_GBs"] / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass"},
+        "tuning": {"seconds": tune_s, **tun.to_json()},
+    }
+    print(json.dumps(out), flush=True)
+    if ident != len(sample) or ident_icl != len(isample):
+        sys.exit(f"{len(sample) - ident} sampled scenarios differ between the batched and the solo run, "
+                 f"{len(isample) - ident_icl} between the batched run and the driver loop")
+
+
+def serve_main(args):
+    from sampo_amd import serve as V
+    from sampo_amd import workloads as W
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    specs = W.serving(rate=args.rate, duration=args.window)
+    offered = sum(sp.n0 * sp.horizon for sp in specs)
+    log(f"serve: {len(specs)} requests over {args.window:.0f} s at {args.rate:.0f}/s, "
+        f"{offered / 1e12:.2f}e12 initial-size agent-steps offered")
+    srv = V.Server(max_slots=args.slots, log=log)
+    results, summ = srv.run(specs, clock="wall")
+    out = {
+        "metric": METRIC, "value": summ["agent_steps_per_s"], "unit": "agent-steps/s", "n_gpus": 1,
+        "steps": summ["days_stepped"], "warmup": 0, "ms_per_step": summ["wall_s"] * 1e3 / max(1, summ["days_stepped"]),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
+        "config": {"workload": "serving: Poisson arrivals, initial population log-uniform 1e4-1e7, horizon "
+                               "30-365 days, temperature/mortality as the sweep, CC = 0.0125 x N0",
+                   "rate_per_s": args.rate, "arrival_window_s": args.window, "requests": len(specs),
+                   "slots": args.slots, "parallelism": "1 GPU"},
+        "latency_s": {"p50": summ.get("p50"), "p99": summ.get("p99"), "mean": summ.get("mean"),
+                      "max": summ.get("max")},
+        "serving": summ,
+        "offered_initial_size_agent_steps": offered,
+    }
+    print(json.dumps(out), flush=True)
+    if summ["completed"] + summ["rejected"] != len(specs):
+        sys.exit("not every request completed")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30, help="timed days")
     ap.add_argument("--warmup", type=int, default=2, help="untimed days before the timed region")
-    ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="large", choices=sorted(WORKLOADS) + ["ensemble", "sweep", "serve"])
+    ap.add_argument("--instances", type=int, default=0, help="batched workloads: scenarios (0: the full set)")
+    ap.add_argument("--plain-sample", type=int, default=64, help="batched workloads: scenarios run alone")
+    ap.add_argument("--icl-sample", type=int, default=16, help="batched workloads: scenarios through the driver loop")
+    ap.add_argument("--no-tune", action="store_true", help="batched workloads: default launch parameters")
+    ap.add_argument("--rate", type=float, default=50.0, help="serve: arrivals per second")
+    ap.add_argument("--window", type=float, default=10.0, help="serve: seconds of arrivals")
+    ap.add_argument("--slots", type=int, default=2048, help="serve: engine slots")
     ap.add_argument("--tile", default="512x1x6", help="work-group size x agents per thread x min waves per SIMD")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-days", type=int, default=2)
@@ -106,6 +214,10 @@ def main():
               "multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload in ("ensemble", "sweep"):
+        return batched_main(args)
+    if args.workload == "serve":
+        return serve_main(args)
     import sampo_amd as S
     w = WORKLOADS[args.workload]
     if args.warmup + args.steps > w["days"]:

@@ -23,6 +23,13 @@ typedef struct {
 	const float *temp;      /* steps temperatures (readSingleValFile, src/abms.c:94-108) */
 	const sampo_env *env;   /* steps environments or 1 (readEnvironment, src/abms.c:110-137) */
 	uint32_t env_count;
+	/* configuration mode (SAMPO_MODE_CONFIG): the given initial population in
+	 * place of initAgents' eggs, and constant per-stage daily mortality dmr
+	 * (egg, larva, pupa, adult) in place of killAgents' formulas */
+	uint32_t mode;
+	float dmr[4];
+	const sampo_layout *init_layout;
+	const sampo_agent *init_agents; /* dense in range order */
 } sampo_abms_params;
 
 typedef struct {

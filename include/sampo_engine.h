@@ -154,6 +154,9 @@ int sampo_engine_time(sampo_engine *e, uint32_t n, int use_graph, float *ms);
  * (synchronises). */
 int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_ms, float *pass_ms);
 
+/* Free and total device memory of HIP device `device`, for admission by footprint. */
+int sampo_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* Device-to-device copy of `bytes` bytes with a plain 16-byte-per-lane
  * kernel, `iters` times; average milliseconds per copy. */
 int sampo_stream_copy(int device, uint64_t bytes, uint32_t iters, float *ms_per_copy);

@@ -101,6 +101,7 @@ def lib():
         L.sampo_engine_agent_steps.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
         L.sampo_engine_time_pass.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.sampo_stream_copy.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]
+        L.sampo_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.sampo_seed_stream.argtypes = [C.c_uint32, C.c_uint32, vp]
         L.sampo_seed_stream.restype = None
         L.sampo_gen_temperature.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float, vp]
@@ -140,6 +141,13 @@ def stream_copy_ms(nbytes, iters=20, device=0):
     ms = C.c_float()
     _check(lib().sampo_stream_copy(device, nbytes, iters, C.byref(ms)))
     return ms.value
+
+
+def device_memory(device=0):
+    """(free, total) bytes of device memory."""
+    f, t = C.c_uint64(), C.c_uint64()
+    _check(lib().sampo_device_memory(device, C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def make_env(bloodmeal=0.25, itn=0.0, irs=0.0, cc=160000.0, larvicide=0.0, ovitrap=0.0):
@@ -300,7 +308,8 @@ ICL_LIB = os.path.join(HERE, "libsampo_icl.so")
 class AbmsParams(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("n_eggs", C.c_uint32), ("max_agents", C.c_uint32), ("steps", C.c_uint32),
                 ("serial_scan", C.c_uint32), ("temp", C.POINTER(C.c_float)), ("env", C.POINTER(Env)),
-                ("env_count", C.c_uint32)]
+                ("env_count", C.c_uint32), ("mode", C.c_uint32), ("dmr", C.c_float * 4),
+                ("init_layout", C.POINTER(Layout)), ("init_agents", C.POINTER(Agent))]
 
 
 class AbmsResult(C.Structure):
@@ -330,7 +339,8 @@ def icl_lib():
     return _icl
 
 
-def run_abms(*, seed, n_eggs, max_agents, steps, temp, env, serial_scan=False, export=True):
+def run_abms(*, seed, n_eggs=0, max_agents, steps, temp, env, serial_scan=False, export=True, mode=MODE_REFERENCE,
+             dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):
     """Run the reference driver loop on the GPU through the icl boundary.
 
     Returns a dict with rows, status, steps_done, agent_steps, layout,
@@ -339,6 +349,17 @@ def run_abms(*, seed, n_eggs, max_agents, steps, temp, env, serial_scan=False, e
     env = np.ascontiguousarray(env)
     p = AbmsParams(seed, n_eggs, max_agents, steps, 1 if serial_scan else 0,
                    temp.ctypes.data_as(C.POINTER(C.c_float)), C.cast(env.ctypes.data, C.POINTER(Env)), env.shape[0])
+    p.mode = mode
+    p.dmr = (C.c_float * 4)(*dmr)
+    keep = None
+    if layout is not None:
+        lay = Layout()
+        for s in range(8):
+            lay.start[s], lay.end[s] = int(layout[s][0]), int(layout[s][1])
+        agents = np.ascontiguousarray(agents, dtype=AGENT_DTYPE)
+        p.init_layout = C.pointer(lay)
+        p.init_agents = C.cast(agents.ctypes.data, C.POINTER(Agent))
+        keep = (lay, agents)
     rows = np.zeros(steps, dtype=ROW_DTYPE)
     r = AbmsResult()
     r.rows = rows.ctypes.data
@@ -348,6 +369,7 @@ def run_abms(*, seed, n_eggs, max_agents, steps, temp, env, serial_scan=False, e
         st = np.zeros(max_agents, dtype=ICL_STATE)
         r.agents, r.ages, r.states, r.span_capacity = ag.ctypes.data, ge.ctypes.data, st.ctypes.data, max_agents
     rc = icl_lib().sampo_abms_run(C.byref(p), C.byref(r))
+    del keep
     if rc != 0:
         raise SampoError(f"sampo_abms_run failed with {rc}")
     layout = np.array([[r.layout.start[s], r.layout.end[s]] for s in range(8)], dtype=np.uint32)

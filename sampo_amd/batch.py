@@ -1,0 +1,236 @@
+"""Batched runs of many scenarios, the plain one-scenario-at-a-time paths
+they are measured against, and the autotuner that picks launch parameters
+per initial-size bucket.
+
+Batched: every scenario is an instance of one engine; each iteration is one
+k_plan + one persistent k_pass + one k_prep launch over all of them, and a
+day is one HIP graph of 24 such iterations.  Between days the host polls the
+instances and grows those that outgrow their buffers
+(sampo_engine_service).
+
+Plain paths, one scenario at a time:
+  engine  the same fused kernels with one instance per engine (the
+          single-run configuration of the large workload);
+  icl     the reference's driver loop over the icl boundary
+          (sampo_abms_run): the reference's kernel sequence and launch
+          widths, a blocking read of the population every iteration, AoS
+          buffers sized to the scenario's capacity.
+
+Autotuner: initial sizes fall in buckets of a factor of 4 from 1e3.  A
+uniform sample of the set large enough to fill the GPU (~3e8 agents) is
+admitted for each trial and timed over one day after a warm-up day.  For
+each bucket in turn (largest share of agents first) its instances' chunk
+size -- agents per work unit of the pass and per output run -- is set to
+each candidate with the other buckets' kept, and the fastest in
+agent-steps/s is kept; then the tile shape of the pass (work-group size,
+agents per thread, register cap), which is one per launch, is chosen the
+same way.  Instances of a bucket are admitted with its chunk size.
+"""
+import json
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import sampo_amd as S
+from sampo_amd import workloads as W
+
+BUCKET_BASE = 1e3
+CHUNK_CANDIDATES = (4096, 16384, 65536, 262144)
+TILE_CANDIDATES = ((512, 1, 6), (256, 1, 8), (512, 2, 0), (1024, 1, 6))
+DEFAULT_CHUNK = 16384
+NO_CHUNK_CAP = 1 << 24  # target chunks per instance: effectively unlimited
+
+
+def bucket_of(n0):
+    return max(0, int(math.floor(math.log(max(n0, 1) / BUCKET_BASE, 4))))
+
+
+def bucket_range(b):
+    return BUCKET_BASE * 4 ** b, BUCKET_BASE * 4 ** (b + 1)
+
+
+@dataclass
+class Tuning:
+    tile: tuple = (512, 1, 6)
+    chunk: dict = field(default_factory=dict)  # bucket -> agents per chunk
+    table: list = field(default_factory=list)  # measurements
+
+    def chunk_for(self, n0):
+        return self.chunk.get(bucket_of(n0), DEFAULT_CHUNK)
+
+    def to_json(self):
+        return {"tile": list(self.tile), "chunk": {str(k): v for k, v in sorted(self.chunk.items())},
+                "table": self.table}
+
+
+def admit_specs(engine, specs, tuning, phys_factor=0.0, kws=None):
+    """Admit specs in order with the tuned chunk size of each one's bucket;
+    returns the slots.  kws: already materialised admit kwargs (else
+    generated on a thread pool)."""
+    slots = []
+    src = zip(specs, kws) if kws is not None else W.materialise_all(specs, phys_factor)
+    for sp, kw in src:
+        engine.set_chunk(tuning.chunk_for(sp.n0), NO_CHUNK_CAP)
+        slots.append(engine.admit(**kw))
+    return slots
+
+
+def _day_rate(engine, days=1):
+    """Agent-steps per second over `days` graph-launched days (one warm-up
+    day first builds the graph and fills the caches)."""
+    engine.step_days(1)
+    engine.service(1.25)
+    engine.agent_steps(reset=True)
+    ms = engine.time(24 * days, graph=True)
+    return engine.agent_steps() / (ms / 1e3), ms
+
+
+def tuning_mix(specs, agents=300_000_000, max_instances=1024):
+    """A uniform sample of the set with about `agents` initial agents --
+    enough to fill the GPU, so that each candidate is timed in the regime
+    the full batch runs in."""
+    total = sum(sp.n0 for sp in specs)
+    k = max(1, int(math.ceil(total / agents)))
+    mix = specs[::k]
+    if len(mix) > max_instances:
+        mix = mix[::int(math.ceil(len(mix) / max_instances))]
+    return mix
+
+
+def _mix_rate(mix, kws, tuning):
+    with S.Engine(max_slots=len(mix)) as e:
+        e.set_tile(*tuning.tile)
+        admit_specs(e, mix, tuning, kws=kws)
+        return _day_rate(e)
+
+
+def autotune(specs, chunks=CHUNK_CANDIDATES, tiles=TILE_CANDIDATES, mix_agents=300_000_000, log=None):
+    """Coordinate descent on a mix of the set: for each size bucket, largest
+    share of agents first, the chunk size of that bucket's instances is set
+    to each candidate while the others keep theirs, and the fastest day in
+    agent-steps/s wins; then the tile shape of the pass the same way."""
+    mix = tuning_mix(specs, mix_agents)
+    kws = [kw for _, kw in W.materialise_all(mix)]
+    share = {}
+    for sp in mix:
+        share[bucket_of(sp.n0)] = share.get(bucket_of(sp.n0), 0) + sp.n0
+    tun = Tuning(chunk={b: DEFAULT_CHUNK for b in share})
+    best_rate, _ = _mix_rate(mix, kws, tun)
+    tun.table.append({"mix": len(mix), "start": True, "agent_steps_per_s": best_rate})
+    if log:
+        log(f"autotune: mix of {len(mix)} scenarios, {sum(share.values()) / 1e6:.0f}M agents: "
+            f"{best_rate:.3e} agent-steps/s at chunk {DEFAULT_CHUNK}")
+    for b in sorted(share, key=lambda x: -share[x]):
+        for c in chunks:
+            if c == tun.chunk[b]:
+                continue
+            trial = Tuning(tile=tun.tile, chunk={**tun.chunk, b: c})
+            rate, ms = _mix_rate(mix, kws, trial)
+            tun.table.append({"bucket": b, "n0_range": bucket_range(b), "chunk": c, "agent_steps_per_s": rate,
+                              "day_ms": ms})
+            if log:
+                log(f"autotune bucket {b} chunk {c}: {rate:.3e} agent-steps/s")
+            if rate > best_rate:
+                best_rate, tun.chunk[b] = rate, c
+    for tile in tiles:
+        if tuple(tile) == tuple(tun.tile):
+            continue
+        trial = Tuning(tile=tuple(tile), chunk=dict(tun.chunk))
+        rate, ms = _mix_rate(mix, kws, trial)
+        tun.table.append({"tile": list(tile), "agent_steps_per_s": rate, "day_ms": ms})
+        if log:
+            log(f"autotune tile {tile}: {rate:.3e} agent-steps/s")
+        if rate > best_rate:
+            best_rate, tun.tile = rate, tuple(tile)
+    return tun
+
+
+def run_batched(specs, tuning, warm_days=1, timed_days=2, phys_factor=1.25, headroom=1.25, keep_rows=(), log=None):
+    """All specs in one engine; times `timed_days` graph-launched days after
+    `warm_days`, growing instances between days.  Returns measurements and
+    the rows of the slots listed in keep_rows (spec indices)."""
+    free0, total = S.device_memory()
+    t_admit = time.perf_counter()
+    with S.Engine(max_slots=len(specs)) as e:
+        e.set_tile(*tuning.tile)
+        slots = admit_specs(e, specs, tuning, phys_factor)
+        e.sync()
+        admit_s = time.perf_counter() - t_admit
+        free1, _ = S.device_memory()
+        if log:
+            log(f"batched: admitted {len(specs)} instances in {admit_s:.1f} s, "
+                f"{(free0 - free1) / 2**30:.1f} GiB resident")
+        for _ in range(warm_days):
+            e.step_days(1)
+            e.service(headroom)
+        e.agent_steps(reset=True)
+        grown = 0
+        t0 = time.perf_counter()
+        for _ in range(timed_days):
+            e.step_days(1)
+            grown += e.service(headroom)  # synchronises: part of the timed day
+        dt = time.perf_counter() - t0
+        steps = e.agent_steps()
+        # the fused pass alone over one more day, for the bandwidth figure
+        e.agent_steps(reset=True)
+        wall_ms, pass_ms = e.time_pass(24)
+        pass_steps = e.agent_steps()
+        st, step, na = e.poll(len(specs))
+        free2, _ = S.device_memory()
+        rows = {i: e.rows(slots[i], 0, int(step[i]) + 1) for i in keep_rows}
+    status = {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))}
+    return {"instances": len(specs), "agent_steps": steps, "seconds": dt, "agent_steps_per_s": steps / dt,
+            "day_ms": dt * 1e3 / timed_days, "grown": grown, "admit_s": admit_s,
+            "resident_gib": (free0 - free2) / 2**30, "status": status, "agents_now": int(na.sum()),
+            "pass_ms_per_iteration": pass_ms / 24, "pass_agent_steps": pass_steps,
+            "pass_GBs": 68.0 * pass_steps / (pass_ms / 1e3) / 1e9}, rows
+
+
+def run_plain_engine(specs, days=1, warm_days=1, keep_rows=True):
+    """Each spec alone in its own engine (default single-run launch
+    parameters); agent-steps/s over the sample and each run's rows."""
+    steps = 0
+    secs = 0.0
+    rows = []
+    for sp, kw in W.materialise_all(specs):
+        with S.Engine(max_slots=1) as e:
+            slot = e.admit(**kw)
+            for _ in range(warm_days):
+                e.step_days(1)
+            e.agent_steps(reset=True)
+            ms = e.time(24 * days, graph=True)
+            steps += e.agent_steps()
+            secs += ms / 1e3
+            if keep_rows:
+                rows.append(e.rows(slot, 0, 24 * (warm_days + days) + 1))
+    return {"instances": len(specs), "agent_steps": steps, "seconds": secs,
+            "agent_steps_per_s": steps / secs if secs else 0.0}, rows
+
+
+def run_plain_icl(specs, days=1):
+    """Each spec alone through the reference's driver loop; agent-steps/s
+    over the sample and each run's rows."""
+    steps = 0
+    secs = 0.0
+    rows = []
+    for sp, kw in W.materialise_all(specs):
+        r = S.run_abms(seed=kw["seed"], max_agents=sp.max_agents, steps=24 * days, temp=kw["temp"], env=kw["env"],
+                       mode=S.MODE_CONFIG, dmr=kw["dmr"], layout=kw["layout"], agents=kw["agents"], export=False)
+        steps += int(r["agent_steps"])
+        secs += r["loop_ms"] / 1e3
+        rows.append(r["rows"])
+    return {"instances": len(specs), "agent_steps": steps, "seconds": secs,
+            "agent_steps_per_s": steps / secs if secs else 0.0}, rows
+
+
+def same_rows(a, b, n):
+    """Rows 0..n-1 of two runs are identical (every printStats field)."""
+    n = min(n, len(a), len(b))
+    return bool(np.array_equal(a[:n], b[:n]))
+
+
+def save_tuning(tuning, path):
+    with open(path, "w") as f:
+        json.dump(tuning.to_json(), f, indent=1)

@@ -17,6 +17,7 @@
 #include "../../include/sampo_icl.h"
 #include "../../include/sampo_abms.h"
 #include "sampo_canon.h"
+#include "sampo_model.h"
 
 #define LOCAL_SIZE 64u
 #define NSEEDS 4u
@@ -39,6 +40,7 @@ typedef struct {
 	cl_uint maxAgents;
 	icl_kernel *calcL1dePerGroup, *calcL1deTotal, *calcFemalesPerGroup, *calcFemalesTotal;
 	icl_kernel *killAgents, *updateAgents, *oldToNewAgents, *preScan, *serialScan, *createEggs;
+	float hmr[4]; /* configuration mode: hourly mortality egg, larva, pupa, adult */
 } Driver;
 
 static size_t roundup(size_t v, size_t m) { return ((v + m - 1) / m) * m; }
@@ -55,11 +57,59 @@ static void generate_seeds(Driver *d, icl_buffer *seedsD, Seeds *seedsH)
 	icl_write_buffer(seedsD, CL_TRUE, NSEEDS * sizeof(Seeds), seedsH, NULL, NULL);
 }
 
+/* Configuration mode: the given agents written into the reference's
+ * array-of-structs buffers at their padded indices (agent.h:66-102); gap
+ * slots are zero and never valid. */
+static void upload_population(Driver *d, icl_buffer *agents, icl_buffer *ages, icl_buffer *states, Population *pop)
+{
+	const sampo_layout *L = d->p->init_layout;
+	const cl_uint span = L->end[7];
+	typedef struct { cl_uint availableEggs, cycleLength, humanBloodmealCount; float delay; cl_uint numEggBatches, aux; } A;
+	typedef struct { float ageInHours, hoursInState; cl_uint isFemale; float csd; } G;
+	typedef struct { cl_uint dead, state; } St;
+	A *ha = (A *)calloc(span ? span : 1, sizeof(A));
+	G *hg = (G *)calloc(span ? span : 1, sizeof(G));
+	St *hs = (St *)calloc(span ? span : 1, sizeof(St));
+	ICL_ASSERT(ha && hg && hs, "host allocation failed");
+	uint64_t k = 0;
+	for (int s = 0; s < 8; ++s) {
+		pop->r[s].start = L->start[s];
+		pop->r[s].end = L->end[s];
+		for (cl_uint g = L->start[s]; g < L->end[s]; ++g, ++k) {
+			const sampo_agent *a = &d->p->init_agents[k];
+			ha[g].availableEggs = a->availableEggs;
+			ha[g].cycleLength = a->cycleLength;
+			ha[g].humanBloodmealCount = a->humanBloodmealCount;
+			ha[g].delay = a->delay;
+			ha[g].numEggBatches = a->numEggBatches;
+			ha[g].aux = a->aux;
+			hg[g].ageInHours = a->ageInHours;
+			hg[g].hoursInState = a->hoursInState;
+			hg[g].isFemale = a->isFemale;
+			hg[g].csd = a->csd;
+			hs[g].dead = 0u;
+			hs[g].state = 1u << s;
+		}
+	}
+	if (span) {
+		icl_write_buffer(agents, CL_TRUE, (size_t)span * sizeof(A), ha, NULL, NULL);
+		icl_write_buffer(ages, CL_TRUE, (size_t)span * sizeof(G), hg, NULL, NULL);
+		icl_write_buffer(states, CL_TRUE, (size_t)span * sizeof(St), hs, NULL, NULL);
+	}
+	free(ha);
+	free(hg);
+	free(hs);
+}
+
 /* src/abms.c:185-228 */
 static void create_initial_population(Driver *d, icl_buffer *agents, icl_buffer *ages, icl_buffer *states,
                                       Population *pop, icl_buffer *enbD, icl_buffer *seeds)
 {
 	d->createEggs = icl_create_kernel(d->dev, "kernel/initAgents.cl", "initAgents", "", ICL_SOURCE);
+	if (d->p->mode == SAMPO_MODE_CONFIG) {
+		upload_population(d, agents, ages, states, pop);
+		return;
+	}
 	size_t local = LOCAL_SIZE;
 	size_t global = roundup(d->maxAgents, LOCAL_SIZE);
 	float t0 = d->p->temp[0];
@@ -114,9 +164,15 @@ static void update(Driver *d, icl_buffer *agents, icl_buffer *ages, icl_buffer *
 	float cc = env->carryingCapacity;
 	float t = d->p->temp[i];
 	float dt = 1.0f;
-	icl_run_kernel(d->killAgents, 1, &global, &local, NULL, NULL, 7, (size_t)0, (void *)ages, (size_t)0,
-	               (void *)states, (size_t)0, (void *)popD, (size_t)0, (void *)enbD, (size_t)0, (void *)bnc,
-	               (size_t)0, (void *)seeds, sizeof(float), &cc);
+	if (d->p->mode == SAMPO_MODE_CONFIG)
+		icl_run_kernel(d->killAgents, 1, &global, &local, NULL, NULL, 10, (size_t)0, (void *)ages, (size_t)0,
+		               (void *)states, (size_t)0, (void *)popD, (size_t)0, (void *)enbD, (size_t)0, (void *)bnc,
+		               (size_t)0, (void *)seeds, sizeof(float), &d->hmr[0], sizeof(float), &d->hmr[1],
+		               sizeof(float), &d->hmr[2], sizeof(float), &d->hmr[3]);
+	else
+		icl_run_kernel(d->killAgents, 1, &global, &local, NULL, NULL, 7, (size_t)0, (void *)ages, (size_t)0,
+		               (void *)states, (size_t)0, (void *)popD, (size_t)0, (void *)enbD, (size_t)0, (void *)bnc,
+		               (size_t)0, (void *)seeds, sizeof(float), &cc);
 	icl_run_kernel(d->updateAgents, 1, &global, &local, NULL, NULL, 11, (size_t)0, (void *)agents, (size_t)0,
 	               (void *)ages, (size_t)0, (void *)states, (size_t)0, (void *)popD, sizeof(sampo_env), (void *)env,
 	               sizeof(float), &t, (size_t)0, (void *)bnc, (size_t)0, (void *)enbD, (size_t)0, (void *)seeds,
@@ -192,10 +248,19 @@ static void swap(icl_buffer **a, icl_buffer **b)
 int sampo_abms_run(const sampo_abms_params *p, sampo_abms_result *res)
 {
 	if (!p || !res || !p->temp || !p->env || p->steps == 0 || p->max_agents < 2 * LOCAL_SIZE) return SAMPO_E_ARG;
+	if (p->mode != SAMPO_MODE_REFERENCE && p->mode != SAMPO_MODE_CONFIG) return SAMPO_E_ARG;
+	if (p->mode == SAMPO_MODE_CONFIG) {
+		const sampo_layout *L = p->init_layout;
+		if (!L || !p->init_agents || L->start[0] != 0) return SAMPO_E_ARG;
+		for (int s = 0; s < 8; ++s)
+			if (L->end[s] < L->start[s] || (s > 0 && L->start[s] <= L->end[s - 1])) return SAMPO_E_ARG;
+		if ((uint64_t)L->end[7] + 8u * LOCAL_SIZE > p->max_agents) return SAMPO_E_ARG;
+	}
 	Driver D;
 	memset(&D, 0, sizeof(D));
 	D.p = p;
 	D.maxAgents = p->max_agents;
+	for (int c = 0; c < 4; ++c) D.hmr[c] = sm_hmr(p->dmr[c]);
 	sc_glibc_seed(&D.rng, p->seed);
 	res->status = 0;
 	res->steps_done = 0;
@@ -208,7 +273,8 @@ int sampo_abms_run(const sampo_abms_params *p, sampo_abms_result *res)
 	icl_start_timer(total);
 
 	/* main(), src/abms.c:606-672 */
-	Enb enbH = {(cl_int)p->n_eggs, p->n_eggs};
+	const cl_uint eggs0 = p->mode == SAMPO_MODE_CONFIG ? 0u : p->n_eggs;
+	Enb enbH = {(cl_int)eggs0, eggs0};
 	Bnc bncH = {0, 0, 0, 0};
 	icl_buffer *seedsD = icl_create_buffer(dev, CL_MEM_READ_ONLY, NSEEDS * sizeof(Seeds));
 	Seeds seedsH[NSEEDS];
@@ -240,7 +306,8 @@ int sampo_abms_run(const sampo_abms_params *p, sampo_abms_result *res)
 	D.calcL1deTotal = icl_create_kernel(dev, "kernel/calcL1de.cl", "calcL1deTotal", "", ICL_SOURCE);
 	D.calcFemalesPerGroup = icl_create_kernel(dev, "kernel/calcGender.cl", "calcFemalesPerGroup", "", ICL_SOURCE);
 	D.calcFemalesTotal = icl_create_kernel(dev, "kernel/calcGender.cl", "calcFemalesTotal", "", ICL_SOURCE);
-	D.killAgents = icl_create_kernel(dev, "kernel/killAgents.cl", "killAgents", "", ICL_SOURCE);
+	D.killAgents = icl_create_kernel(dev, "kernel/killAgents.cl",
+	                                 p->mode == SAMPO_MODE_CONFIG ? "killAgentsStaged" : "killAgents", "", ICL_SOURCE);
 	D.updateAgents = icl_create_kernel(dev, "kernel/updateAgents.cl", "updateAgents", "", ICL_SOURCE);
 	D.oldToNewAgents = icl_create_kernel(dev, "kernel/oldToNewAgents.cl", "oldToNewAgents", "", ICL_SOURCE);
 	D.serialScan = icl_create_kernel(dev, "kernel/serialScan.cl", "serialScan", "", ICL_SOURCE);

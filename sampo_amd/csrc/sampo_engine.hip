@@ -1855,6 +1855,16 @@ __global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ s
 		dst[i] = src[i];
 }
 
+extern "C" int sampo_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes)
+{
+	HIPCHK(hipSetDevice(device));
+	size_t f = 0, t = 0;
+	HIPCHK(hipMemGetInfo(&f, &t));
+	if (free_bytes) *free_bytes = f;
+	if (total_bytes) *total_bytes = t;
+	return 0;
+}
+
 extern "C" int sampo_stream_copy(int device, uint64_t bytes, uint32_t iters, float *ms_per_copy)
 {
 	if (!ms_per_copy || bytes < 16 || iters == 0) return fail(SAMPO_E_ARG, "bad arguments%s%ld");

@@ -84,6 +84,13 @@ void l_killAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
 	hipLaunchKernelGGL(killAgents, g, b, 0, s, B(IclAgentAge, 0), B(IclAgentState, 1), B(const IclPopulation, 2),
 	                   B(IclEnb, 3), B(IclBnc, 4), B(const IclSeeds, 5), V(float, 6));
 }
+void l_killAgentsStaged(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+{
+	const char *name = "killAgentsStaged";
+	hipLaunchKernelGGL(killAgentsStaged, g, b, 0, s, B(IclAgentAge, 0), B(IclAgentState, 1),
+	                   B(const IclPopulation, 2), B(IclEnb, 3), B(IclBnc, 4), B(const IclSeeds, 5), V(float, 6),
+	                   V(float, 7), V(float, 8), V(float, 9));
+}
 void l_updateAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
 {
 	const char *name = "updateAgents";
@@ -154,6 +161,7 @@ struct Entry {
 const Entry k_registry[] = {
 	{"initAgents.cl", "initAgents", 6, l_initAgents},
 	{"killAgents.cl", "killAgents", 7, l_killAgents},
+	{"killAgents.cl", "killAgentsStaged", 10, l_killAgentsStaged}, /* configuration mode, not a reference kernel */
 	{"updateAgents.cl", "updateAgents", 11, l_updateAgents},
 	{"calcL1de.cl", "calcL1dePerGroup", 3, l_calcL1dePerGroup},
 	{"calcL1de.cl", "calcL1deTotal", 3, l_calcL1deTotal},

@@ -96,6 +96,30 @@ __global__ void killAgents(IclAgentAge *agents, IclAgentState *states, const Icl
 	if (u <= rate) states[g].dead = 1u;
 }
 
+/* Configuration-mode mortality: the range classes of killAgents.cl:37-147
+ * (adults from immatures.start, eggs below eggs.end, larvae below
+ * larvae.end, pupae otherwise) with one hourly rate per class in place of
+ * the reference formulas.  An addition of this library, used by the
+ * driver loop for scenarios with constant per-stage daily mortality. */
+__global__ void killAgentsStaged(IclAgentAge *agents, IclAgentState *states, const IclPopulation *pop, IclEnb *enb,
+                                 IclBnc *bnc, const IclSeeds *seeds, float hmrEgg, float hmrLarva, float hmrPupa,
+                                 float hmrAdult)
+{
+	uint32_t g = gid1();
+	if (g == 0) {
+		enb->newEggs = 0u;
+		enb->totalBiomass = (int32_t)(pop->numLarvae1DayEquiv + pop->r[0].end + (pop->r[2].end - pop->r[2].start));
+		bnc->numBites = 0u;
+		bnc->numInfectBites = 0u;
+		bnc->numCycles = 0u;
+		bnc->sumCycles = 0u;
+	}
+	if (!in_range(pop, g)) return;
+	float u = taus(&seeds[0], g);
+	float rate = g >= pop->r[3].start ? hmrAdult : g < pop->r[0].end ? hmrEgg : g < pop->r[1].end ? hmrLarva : hmrPupa;
+	if (u <= rate) states[g].dead = 1u;
+}
+
 /* include/properties.h:257-266 */
 __device__ __attribute__((noinline)) static uint32_t gen_eggs(float u0, float u1, uint32_t nb)
 {

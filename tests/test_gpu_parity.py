@@ -215,12 +215,19 @@ def test_engine_status_capacity_and_extinction():
     assert sim.status == 1 and sim.step == step
 
 
-def icl_vs_oracle(n_eggs, cc, steps, seed, serial):
-    temp, env = ref_inputs(steps, cc)
+def icl_vs_oracle(n_eggs, cc, steps, seed, serial, config=None):
     cap = 400_000
-    sim = O.Sim(temp=temp, env=env, capacity=cap, seed=seed, n_eggs=n_eggs)
+    if config is None:
+        temp, env = ref_inputs(steps, cc)
+        kw = dict(n_eggs=n_eggs)
+    else:
+        n0, dmr = config
+        temp, lay, ag = config_inputs(n0, (steps + 23) // 24, seed)
+        env = S.make_env(cc=cc)
+        kw = dict(mode=S.MODE_CONFIG, dmr=dmr, layout=lay, agents=ag)
+    sim = O.Sim(temp=temp, env=env, capacity=cap, seed=seed, **kw)
     orows = sim.run(steps)
-    g = S.run_abms(seed=seed, n_eggs=n_eggs, max_agents=cap, steps=steps, temp=temp, env=env, serial_scan=serial)
+    g = S.run_abms(seed=seed, max_agents=cap, steps=steps, temp=temp, env=env, serial_scan=serial, **kw)
     assert g["status"] == 0 and g["steps_done"] == steps
     rows_equal(orows, g["rows"])
     lay_o = sim.layout()
@@ -233,6 +240,12 @@ def icl_vs_oracle(n_eggs, cc, steps, seed, serial):
         assert np.array_equal(g["ages"][f][idx], ag_o[f]), f
     assert np.array_equal(g["states"]["state"][idx], st_o)
     assert np.array_equal(g["histogram"], sim.histogram())
+
+
+def test_icl_driver_configuration_mode_matches_oracle():
+    """The plain driver loop with an explicit initial population and
+    constant per-stage mortality (the killAgentsStaged kernel)."""
+    icl_vs_oracle(0, 150.0, 24 * 8, 17, serial=False, config=(6000, (0.05, 0.03, 0.02, 0.1)))
 
 
 def test_icl_driver_parallel_scan_matches_oracle():
