@@ -205,6 +205,10 @@ def main():
     ap.add_argument("--window", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=2048, help="serve: engine slots")
     ap.add_argument("--tile", default="512x1x6", help="work-group size x agents per thread x min waves per SIMD")
+    ap.add_argument("--per-cu", type=int, default=0, help="persistent work-groups of the pass per CU (0: all that fit)")
+    ap.add_argument("--chunk", type=int, default=4096, help="single run: minimum agents per chunk")
+    ap.add_argument("--target-chunks", type=int, default=0,
+                    help="single run: most chunks per iteration (0: two per resident work-group)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-days", type=int, default=2)
     ap.add_argument("--cpu-scale", type=int, default=10)
@@ -226,6 +230,8 @@ def main():
     wg, apt, waves = (int(x) for x in (args.tile.split("x") + ["0"])[:3])
     with S.Engine(max_slots=1) as e:
         e.set_tile(wg, apt, waves)
+        e.set_grid(args.per_cu)
+        e.set_chunk(args.chunk, args.target_chunks)
         t_admit = time.perf_counter()
         slot = e.admit(mode=S.MODE_CONFIG, seed=w["seed"], horizon=24 * w["days"], temp=temp, env=env,
                        max_agents=w["max_agents"], dmr=DMR, layout=layout, agents=agents)
@@ -261,7 +267,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+u32", "data": "synthetic",
         "config": {"workload": w["desc"], "initial_agents": w["n0"], "max_agents": w["max_agents"],
                    "carrying_capacity": w["cc_per_agent"] * w["n0"], "timed_days": args.steps,
-                   "iterations_per_day": 24, "tile": args.tile, "agents_at_end": n_now,
+                   "iterations_per_day": 24, "tile": args.tile, "chunk": args.chunk,
+                   "target_chunks": args.target_chunks, "agents_at_end": n_now,
                    "mean_agents": steps_done / n_iter, "parallelism": "1 GPU"},
         "agent_days_per_s": value / 24.0,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -120,6 +120,10 @@ void sampo_engine_destroy(sampo_engine *e);
  * shapes compiled into the library are accepted (SAMPO_E_ARG otherwise). */
 int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt, uint32_t waves);
 
+/* Persistent work-groups of the pass per CU (0: as many as the tile shape
+ * lets reside; otherwise at most that many). */
+int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu);
+
 /* Chunking of instances admitted from now on.  A chunk is the run of agents
  * one work-group takes and the granularity of output segments: at least
  * `agents` agents (default 4096), grown so that an instance has at most

@@ -83,6 +83,7 @@ def lib():
         L.sampo_engine_destroy.restype = None
         L.sampo_engine_set_tile.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32]
         L.sampo_engine_set_chunk.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.sampo_engine_set_grid.argtypes = [vp, C.c_uint32]
         L.sampo_engine_admit.argtypes = [vp, C.POINTER(Scenario), C.POINTER(C.c_uint32)]
         L.sampo_engine_release.argtypes = [vp, C.c_uint32]
         L.sampo_engine_footprint.argtypes = [C.POINTER(Scenario), C.c_uint32]
@@ -216,6 +217,10 @@ class Engine:
     def set_tile(self, wg, apt, waves=0):
         """Work-group size, agents per thread, register cap (min waves/SIMD)."""
         _check(lib().sampo_engine_set_tile(self.h, wg, apt, waves))
+
+    def set_grid(self, per_cu=0):
+        """Persistent work-groups of the pass per CU (0: all that fit)."""
+        _check(lib().sampo_engine_set_grid(self.h, per_cu))
 
     def set_chunk(self, agents, target_chunks=0):
         """Minimum agents per chunk (work unit / output segment) and the most

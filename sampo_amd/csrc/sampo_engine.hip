@@ -1165,6 +1165,7 @@ struct sampo_engine {
 	uint32_t target_chunks = 0; /* 0: 2 per resident work-group of the pass */
 	pass_fn pass = nullptr;
 	int grid = 0;
+	uint32_t per_cu = 0; /* persistent work-groups per CU; 0: as many as fit */
 	hipGraphExec_t day = nullptr;
 	hipGraph_t day_graph = nullptr;
 };
@@ -1192,6 +1193,7 @@ static int configure_pass(sampo_engine *e)
 	HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)e->pass, (int)e->wg, 0));
 	HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
 	if (blocks < 1) blocks = 1;
+	if (e->per_cu && (int)e->per_cu < blocks) blocks = (int)e->per_cu;
 	e->grid = blocks * cus;
 	return 0;
 }
@@ -1259,6 +1261,16 @@ extern "C" int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt,
 	e->apt = apt;
 	e->waves = waves;
 	e->pass = f;
+	return configure_pass(e);
+}
+
+extern "C" int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu)
+{
+	if (!e || per_cu > 64u) return fail(SAMPO_E_ARG, "per_cu must be 0..64%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	drop_graph(e);
+	e->per_cu = per_cu;
 	return configure_pass(e);
 }
 
@@ -1876,7 +1888,7 @@ extern "C" int sampo_stream_copy(int device, uint64_t bytes, uint32_t iters, flo
 	HIPCHK(hipMemset(a, 1, n * 16));
 	int cus = 0;
 	HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-	dim3 grid((unsigned)(cus * 16));
+	dim3 grid((unsigned)(cus * 4)); /* 4 groups of 256 per CU: the fastest plain copy measured */
 	hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, 0, a, b, n);
 	hipEvent_t x, y;
 	HIPCHK(hipEventCreate(&x));

@@ -77,6 +77,60 @@ def test_engine_config_mode_matches_oracle(n0, days, seed, dmr):
         agents_equal(ag_o, ag_g)
 
 
+def intervention_env(steps, cc, seed):
+    """A time-varying environment with every intervention switched on in
+    some hours: larvicide, ITN, IRS and ovitrap coverage x effectiveness
+    (updateAgents.cl:90-97, 166-173, 200-207, 236-264)."""
+    rng = np.random.default_rng(seed)
+    env = np.repeat(S.make_env(cc=cc), steps)
+    env["bloodmealSuccess"] = rng.uniform(0.2, 0.9, steps).astype(np.float32)
+    for f, hi in (("ITNValue", 0.4), ("IRSValue", 0.2), ("larvacideValue", 0.3), ("oviTrapValue", 0.5)):
+        v = rng.uniform(0.0, hi, steps).astype(np.float32)
+        v[rng.uniform(size=steps) < 0.3] = 0.0  # some hours without the intervention
+        env[f] = v
+    return env
+
+
+@pytest.mark.parametrize("mode", ["reference", "config"])
+def test_engine_interventions_match_oracle(mode):
+    days, seed = (20 if mode == "reference" else 12), 23
+    steps = 24 * days
+    if mode == "reference":
+        temp = np.full(steps, 29.0, np.float32)
+        env = intervention_env(steps, 12000.0, seed)
+        kw = dict(n_eggs=4000)
+        akw = dict(mode=S.MODE_REFERENCE, n_eggs=4000)
+    else:
+        temp, lay, ag = config_inputs(8000, days, seed)
+        env = intervention_env(steps, 0.025 * 8000, seed)
+        dmr = (0.05, 0.03, 0.02, 0.1)
+        kw = dict(layout=lay, agents=ag, mode=1, dmr=dmr)
+        akw = dict(mode=S.MODE_CONFIG, dmr=dmr, layout=lay, agents=ag)
+    sim = O.Sim(temp=temp, env=env, capacity=400_000, seed=seed, **kw)
+    orows = sim.run(steps)
+    assert orows["count"][-1].sum() > 0
+    with S.Engine(max_slots=1) as e:
+        slot = e.admit(seed=seed, horizon=steps, temp=temp, env=env, max_agents=400_000, **akw)
+        e.step(steps)
+        rows_equal(orows, e.rows(slot, 0, steps))
+        lay_o, ag_o, _ = sim.export()
+        lay_g, ag_g = e.export(slot)
+        assert np.array_equal(lay_o, lay_g)
+        agents_equal(ag_o, ag_g)
+
+
+def test_icl_driver_interventions_match_oracle():
+    steps, seed = 24 * 8, 29
+    temp = np.full(steps, 28.0, np.float32)
+    env = intervention_env(steps, 9000.0, seed)
+    sim = O.Sim(temp=temp, env=env, capacity=400_000, seed=seed, n_eggs=3000)
+    orows = sim.run(steps)
+    g = S.run_abms(seed=seed, n_eggs=3000, max_agents=400_000, steps=steps, temp=temp, env=env, export=False)
+    assert g["status"] == 0 and g["steps_done"] == steps
+    rows_equal(orows, g["rows"])
+    assert np.array_equal(sim.layout(), g["layout"])
+
+
 def test_engine_hip_graph_day_equals_plain_steps():
     steps = 24 * 6
     temp, env = ref_inputs(steps, 8000.0)
