@@ -755,6 +755,40 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts)
 
 /* ------------------------------------------------------------ k_pass */
 
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void glob_void;
+/* Global-memory views of pointers the pass reads from LDS: accesses through
+ * them are global (not flat) instructions, which the compiler knows cannot
+ * touch LDS -- so they never make it wait before an LDS access. */
+typedef __attribute__((address_space(1))) uint32_t gu32;
+
+/* Barrier for LDS traffic only.  The pass never exchanges global memory
+ * between the waves of a work-group, and the direct-to-LDS loads of a tile
+ * land in slots only the issuing lane reads (after its own wait on them), so
+ * a barrier waits for the wave's LDS instructions (lgkmcnt) alone -- which
+ * lets the next tile's loads stay in flight across the current tile's
+ * barriers.  The empty asm statements keep the compiler from moving memory
+ * accesses across the barrier. */
+__device__ static inline void lds_barrier()
+{
+	__asm__ __volatile__("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt(0xC07F); /* lgkmcnt(0) */
+	__builtin_amdgcn_s_barrier();
+	__asm__ __volatile__("" ::: "memory");
+}
+
+/* One agent of a tile: where it comes from. */
+struct Src {
+	uint32_t g;     /* virtual index (keys the random numbers) */
+	uint32_t s;     /* state, S_DEAD for no agent */
+	uint32_t born;  /* newborn: regenerated, not loaded */
+};
+
+/* The pass over a chunk is software-pipelined by tiles of WG * APT agents:
+ * while tile t is updated, ranked and written, the loads of tile t + 1 run
+ * as direct-to-LDS loads (global_load_lds_dword) into the other of two LDS
+ * buffers.  Each lane's slots in a buffer are its own: it loads them, reads
+ * them and writes its updated agent back into them. */
 template <int WG, int APT, int WAVES>
 __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const uint32_t *pl_base,
                                              const uint8_t *pl_par, uint32_t ninst)
@@ -766,15 +800,91 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	__shared__ uint32_t s_ticket, s_inst;
 	__shared__ uint32_t s_k0[SEGC], s_st[SEGC], s_pb[SEGC], s_vb[SEGC];
 	__shared__ uint32_t s_wlo, s_wm, s_wend; /* window: global index, size, dense end */
-	__shared__ uint32_t s_wc[APT][NW][16]; /* per (input state, moved) */
+	__shared__ uint32_t s_wc[APT][NW][16];    /* per (input state, moved) */
 	__shared__ uint32_t s_run[16], s_tc[16];
-	__shared__ uint32_t s_head[8], s_end[8]; /* the chunk's block of each input state */
+	__shared__ uint32_t s_head[8], s_end[8];  /* the chunk's block of each input state */
 	__shared__ uint32_t s_acc[NW][8];
-	__shared__ uint32_t s_out[NFIELD][TILE]; /* updated agents of the tile */
+	__shared__ uint32_t bufA[NFIELD][TILE], bufB[NFIELD][TILE];
+	__shared__ uint32_t s_sg[TILE], s_ss[TILE]; /* next tile, per lane: virtual index; state | newborn << 8 */
 
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
 	const uint32_t total = ctx->total;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+
+	/* stage the segments from global index `from` on until the window covers
+	 * dense index te - 1 (wave 0) */
+	auto stage = [&](uint32_t from, uint32_t te) {
+		uint32_t j = from, m = 0, kend = T.ip.n;
+		for (;;) {
+			uint32_t jj = j + lane;
+			Seg sg = {0u, 0u, 0u, 0u};
+			bool valid = jj < T.nseg;
+			if (valid) {
+				const gu32 *q = (const gu32 *)T.seg + (size_t)jj * 4u;
+				sg.k0 = q[0];
+				sg.state = q[1];
+				sg.pbase = q[2];
+				sg.vbase = q[3];
+			}
+			bool in = valid && sg.k0 < te;
+			uint64_t bl = __ballot(in);
+			uint32_t cnt = (uint32_t)__popcll(bl);
+			if (in) {
+				s_k0[m + lane] = sg.k0;
+				s_st[m + lane] = sg.state;
+				s_pb[m + lane] = sg.pbase;
+				s_vb[m + lane] = sg.vbase;
+			}
+			m += cnt;
+			if (cnt < 64u) { /* first lane past te holds the window end */
+				uint64_t past = __ballot(valid && !in);
+				if (past) kend = __shfl(sg.k0, (int)(__ffsll((unsigned long long)past) - 1), 64);
+				break;
+			}
+			if (m + 64u > SEGC) { /* window full: it ends where the next segment starts */
+				uint32_t nk = j + 64u < T.nseg ? ((const gu32 *)T.seg)[(size_t)(j + 64u) * 4u] : T.ip.n;
+				kend = nk;
+				break;
+			}
+			j += 64u;
+		}
+		if (lane == 0) {
+			s_wlo = from;
+			s_wm = m;
+			s_wend = kend;
+		}
+		/* every load above is complete before any lane leaves: nothing
+		 * loaded here stays pending past the next tile's loads */
+		__builtin_amdgcn_s_waitcnt(0x0F70);
+	};
+
+	/* sources of tile [ts, te) from the window, and their loads into buf;
+	 * returns the global index of the segment holding te - 1 */
+	auto issue = [&](uint32_t ts, uint32_t te, uint32_t (&buf)[NFIELD][TILE]) -> uint32_t {
+		const uint32_t m = s_wm;
+#pragma unroll
+		for (int jx = 0; jx < APT; ++jx) {
+			const uint32_t k = ts + (uint32_t)jx * WG + tid;
+			const uint32_t o = (uint32_t)jx * WG + tid;
+			s_ss[o] = S_DEAD;
+			if (k < te) {
+				const uint32_t q = seg_find(s_k0, 0u, m, k);
+				const uint32_t r = k - s_k0[q];
+				const bool born = s_pb[q] == PB_NEWBORN;
+				s_ss[o] = (s_st[q] & 0xffu) | (born ? 0x100u : 0u);
+				s_sg[o] = s_vb[q] + r;
+				if (!born) {
+					const uint32_t p = seg_slot(s_pb[q], s_st[q], r);
+					const gu32 *b = (const gu32 *)T.bin + p;
+#pragma unroll
+					for (int f = 0; f < NFIELD; ++f)
+						__builtin_amdgcn_global_load_lds((glob_void *)(b + (size_t)f * T.P),
+						                                 (lds_void *)&buf[f][(uint32_t)jx * WG + w * 64u], 4, 0, 0);
+				}
+			}
+		}
+		return s_wlo + seg_find(s_k0, 0u, m, te - 1u);
+	};
 
 	for (;;) {
 		if (tid == 0) {
@@ -816,9 +926,6 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				T.cmax = I->cmax;
 				T.P = I->P;
 				T.nseg = I->st[par].nseg;
-				s_wend = 0u; /* force staging at the first tile */
-				s_wlo = 0u;
-				s_wm = 0u;
 			}
 			if (tid < 16u) s_run[tid] = 0u;
 		}
@@ -830,108 +937,88 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			s_end[tid] = z > h ? z : h;
 		}
 		const size_t P = T.P;
+		const uint32_t k0 = T.ch.k0, k1 = T.ch.k1;
 		Acc acc = {0, 0, 0, 0, 0};
 		uint32_t l1de = 0, fem = 0, pinf = 0;
-		uint32_t next = T.ch.seg0; /* global index of the segment holding ts */
 
-		for (uint32_t ts = T.ch.k0; ts < T.ch.k1; ts += TILE) {
-			const uint32_t te = ts + TILE < T.ch.k1 ? ts + TILE : T.ch.k1;
-			if (te > s_wend && w == 0) { /* stage segments from the one holding ts */
-				uint32_t j = next, m = 0, kend = T.ip.n;
-				for (;;) {
-					uint32_t jj = j + lane;
-					Seg sg = {0u, 0u, 0u, 0u};
-					bool valid = jj < T.nseg;
-					if (valid) sg = T.seg[jj];
-					bool in = valid && sg.k0 < te;
-					uint64_t bl = __ballot(in);
-					uint32_t cnt = (uint32_t)__popcll(bl);
-					if (in) {
-						s_k0[m + lane] = sg.k0;
-						s_st[m + lane] = sg.state;
-						s_pb[m + lane] = sg.pbase;
-						s_vb[m + lane] = sg.vbase;
-					}
-					m += cnt;
-					if (cnt < 64u) { /* first lane past te holds the window end */
-						uint64_t past = __ballot(valid && !in);
-						if (past) kend = __shfl(sg.k0, (int)(__ffsll((unsigned long long)past) - 1), 64);
-						break;
-					}
-					if (m + 64u > SEGC) { /* window full: it ends where the next segment starts */
-						uint32_t nk = j + 64u < T.nseg ? T.seg[j + 64u].k0 : T.ip.n;
-						kend = nk;
-						break;
-					}
-					j += 64u;
-				}
-				if (lane == 0) {
-					s_wlo = next;
-					s_wm = m;
-					s_wend = kend;
-				}
-			}
-			__syncthreads();
-			const uint32_t m = s_wm, wlo = s_wlo;
-			/* the input states the tile spans (dense order is state-sorted) */
-			const uint32_t q_hi = seg_find(s_k0, 0u, m, te - 1u);
-			const uint32_t s_lo = s_st[seg_find(s_k0, 0u, m, ts)] & 0xffu;
-			const uint32_t s_hi = s_st[q_hi] & 0xffu;
-			uint32_t ns[APT], lr[APT];
-			/* gather: all of the tile's loads issued before any update */
+		/* prologue: tile 0's window and loads */
+		{
+			const uint32_t te0 = k0 + TILE < k1 ? k0 + TILE : k1;
+			if (w == 0) stage(T.ch.seg0, te0);
+			lds_barrier();
+		}
+		uint32_t next = issue(k0, k0 + TILE < k1 ? k0 + TILE : k1, bufA);
+
+		/* one tile: its agents are in cur (loaded) or regenerated; the next
+		 * tile's loads go to nxt */
+		auto tile = [&](uint32_t ts, uint32_t (&cur)[NFIELD][TILE], uint32_t (&nxt)[NFIELD][TILE]) {
+			const uint32_t te = ts + TILE < k1 ? ts + TILE : k1;
+			Ag a[APT];
+			Src me[APT];
+			/* this tile's loads into cur are the oldest memory operations in
+			 * flight: wait for them (vmcnt(0)) */
+			__builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
 			for (int jx = 0; jx < APT; ++jx) {
-				const uint32_t k = ts + (uint32_t)jx * WG + tid;
 				const uint32_t o = (uint32_t)jx * WG + tid;
-				if (k < te) {
-					const uint32_t q = seg_find(s_k0, 0u, m, k);
-					const uint32_t r = k - s_k0[q];
-					Ag a;
-					if (s_pb[q] == PB_NEWBORN) newborn(a, T.nbseed, T.ip.nb_temp, s_vb[q] + r);
-					else load_agent(a, T.bin, P, seg_slot(s_pb[q], s_st[q], r));
-					s_out[F_AGE][o] = sc_bits_f(a.age);
-					s_out[F_HIS][o] = sc_bits_f(a.his);
-					s_out[F_DLY][o] = sc_bits_f(a.dly);
-					s_out[F_AUX][o] = a.aux;
-					s_out[F_CSD][o] = sc_bits_f(a.csd);
-					s_out[F_AVAIL][o] = a.avail;
-					s_out[F_CYC][o] = a.cyc;
-					s_out[F_MISC][o] = a.misc;
+				const uint32_t ss = s_ss[o];
+				me[jx].s = ss & 0xffu;
+				me[jx].born = ss >> 8;
+				me[jx].g = s_sg[o];
+				if (me[jx].s < S_DEAD) {
+					if (me[jx].born) {
+						newborn(a[jx], T.nbseed, T.ip.nb_temp, me[jx].g);
+					} else {
+						a[jx].age = sc_float(cur[F_AGE][o]);
+						a[jx].his = sc_float(cur[F_HIS][o]);
+						a[jx].dly = sc_float(cur[F_DLY][o]);
+						a[jx].aux = cur[F_AUX][o];
+						a[jx].csd = sc_float(cur[F_CSD][o]);
+						a[jx].avail = cur[F_AVAIL][o];
+						a[jx].cyc = cur[F_CYC][o];
+						a[jx].misc = cur[F_MISC][o];
+					}
 				}
 			}
+			/* the input states the tile spans (dense order is state-sorted) */
+			uint32_t s_lo, s_hi;
+			{
+				const uint32_t m = s_wm;
+				s_lo = s_st[seg_find(s_k0, 0u, m, ts)] & 0xffu;
+				s_hi = s_st[seg_find(s_k0, 0u, m, te - 1u)] & 0xffu;
+			}
+			/* next tile: window (wave 0), barrier, loads into nxt */
+			const uint32_t ts2 = te, te2 = ts2 + TILE < k1 ? ts2 + TILE : k1;
+			const bool more = ts2 < k1;
+			lds_barrier(); /* every lane has read the window for this tile */
+			if (more) {
+				if (w == 0 && te2 > s_wend) stage(next, te2);
+				lds_barrier();
+				next = issue(ts2, te2, nxt);
+			}
+			uint32_t ns[APT], lr[APT];
 #pragma unroll 1
 			for (int jx = 0; jx < APT; ++jx) {
-				const uint32_t k = ts + (uint32_t)jx * WG + tid;
 				uint32_t n2 = 0xffu; /* class 2 * input state + moved, 0xff: dead or no agent */
-				if (k < te) {
-					const uint32_t q = seg_find(s_k0, 0u, m, k);
-					const uint32_t s = s_st[q] & 0xffu;
-					const uint32_t g = s_vb[q] + (k - s_k0[q]);
+				if (me[jx].s < S_DEAD) {
+					const uint32_t s = me[jx].s;
 					const uint32_t o = (uint32_t)jx * WG + tid;
-					Ag a;
-					a.age = sc_float(s_out[F_AGE][o]);
-					a.his = sc_float(s_out[F_HIS][o]);
-					a.dly = sc_float(s_out[F_DLY][o]);
-					a.aux = s_out[F_AUX][o];
-					a.csd = sc_float(s_out[F_CSD][o]);
-					a.avail = s_out[F_AVAIL][o];
-					a.cyc = s_out[F_CYC][o];
-					a.misc = s_out[F_MISC][o];
-					const uint32_t ns2 = agent_step(a, s, g, T, acc);
+					Ag x = a[jx];
+					const uint32_t ns2 = agent_step(x, s, me[jx].g, T, acc);
 					if (ns2 < S_DEAD) {
 						n2 = 2u * s + (ns2 != s ? 1u : 0u);
-						s_out[F_AGE][o] = sc_bits_f(a.age);
-						s_out[F_HIS][o] = sc_bits_f(a.his);
-						s_out[F_DLY][o] = sc_bits_f(a.dly);
-						s_out[F_AUX][o] = a.aux;
-						s_out[F_CSD][o] = sc_bits_f(a.csd);
-						s_out[F_AVAIL][o] = a.avail;
-						s_out[F_CYC][o] = a.cyc;
-						s_out[F_MISC][o] = a.misc;
-						if (ns2 == S_LARVA) l1de += sc_f2u(a.age / 24.0f);
-						if (ns2 >= S_IMM && (a.misc >> 31)) {
+						cur[F_AGE][o] = sc_bits_f(x.age);
+						cur[F_HIS][o] = sc_bits_f(x.his);
+						cur[F_DLY][o] = sc_bits_f(x.dly);
+						cur[F_AUX][o] = x.aux;
+						cur[F_CSD][o] = sc_bits_f(x.csd);
+						cur[F_AVAIL][o] = x.avail;
+						cur[F_CYC][o] = x.cyc;
+						cur[F_MISC][o] = x.misc;
+						if (ns2 == S_LARVA) l1de += sc_f2u(x.age / 24.0f);
+						if (ns2 >= S_IMM && (x.misc >> 31)) {
 							fem += 1u;
-							pinf += a.csd >= 1.0f ? 1u : 0u;
+							pinf += x.csd >= 1.0f ? 1u : 0u;
 						}
 					}
 				}
@@ -946,7 +1033,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				ns[jx] = n2;
 				lr[jx] = rank;
 			}
-			__syncthreads();
+			lds_barrier();
 			if (tid < 16u) { /* exclusive over (jx, wave) per class, after the chunk's earlier tiles */
 				uint32_t run = s_run[tid];
 				if ((tid >> 1) >= s_lo && (tid >> 1) <= s_hi)
@@ -958,9 +1045,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 						}
 				s_tc[tid] = run;
 			}
-			__syncthreads();
-			/* the segment holding te - 1 starts the next tile's lookup */
-			next = wlo + q_hi;
+			lds_barrier();
 #pragma unroll 1
 			for (int jx = 0; jx < APT; ++jx) {
 				const uint32_t cl = ns[jx];
@@ -968,19 +1053,24 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t o = (uint32_t)jx * WG + tid;
 					const uint32_t off = s_wc[jx][w][cl] + lr[jx];
 					const uint32_t p = (cl & 1u) ? s_end[cl >> 1] - 1u - off : s_head[cl >> 1] + off;
-					uint32_t *b = T.bout;
-					b[p] = s_out[F_AGE][o];
-					b[P + p] = s_out[F_HIS][o];
-					b[2 * P + p] = s_out[F_DLY][o];
-					b[3 * P + p] = s_out[F_AUX][o];
-					b[4 * P + p] = s_out[F_CSD][o];
-					b[5 * P + p] = s_out[F_AVAIL][o];
-					b[6 * P + p] = s_out[F_CYC][o];
-					b[7 * P + p] = s_out[F_MISC][o];
+					gu32 *b = (gu32 *)T.bout;
+					b[p] = cur[F_AGE][o];
+					b[P + p] = cur[F_HIS][o];
+					b[2 * P + p] = cur[F_DLY][o];
+					b[3 * P + p] = cur[F_AUX][o];
+					b[4 * P + p] = cur[F_CSD][o];
+					b[5 * P + p] = cur[F_AVAIL][o];
+					b[6 * P + p] = cur[F_CYC][o];
+					b[7 * P + p] = cur[F_MISC][o];
 				}
 			}
-			__syncthreads();
+			/* the next tile rewrites s_wc only after its first barrier */
 			if (tid < 16u) s_run[tid] = s_tc[tid];
+		};
+
+		for (uint32_t ts = k0; ts < k1; ts += 2u * TILE) {
+			tile(ts, bufA, bufB);
+			if (ts + TILE < k1) tile(ts + TILE, bufB, bufA);
 		}
 		{ /* chunk results */
 			uint32_t v[8] = {acc.eggs, acc.bites, acc.ibites, acc.ncyc, acc.scyc, l1de, fem, pinf};
@@ -993,10 +1083,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			if (tid < 8u) {
 				uint32_t r = 0;
 				for (int q = 0; q < NW; ++q) r += s_acc[q][tid];
-				T.cout[(size_t)(CO_SUMS + tid) * T.cmax + c] = r;
+				((gu32 *)T.cout)[(size_t)(CO_SUMS + tid) * T.cmax + c] = r;
 			}
 			if (tid < 16u) /* stayed: row q, moved on: row 8 + q */
-				T.cout[(size_t)((tid & 1u) * 8u + (tid >> 1)) * T.cmax + c] = s_run[tid];
+				((gu32 *)T.cout)[(size_t)((tid & 1u) * 8u + (tid >> 1)) * T.cmax + c] = s_run[tid];
 			if (tid == 0) insts[ii].pending = 1u;
 		}
 		__syncthreads();
