@@ -1,20 +1,19 @@
 /*
  * sampo_engine.hip -- fused, batched per-iteration agent step for gfx950.
  *
- * One iteration of the reference loop (src/abms.c:502-529) is three launches
+ * One iteration of the reference loop (src/abms.c:502-529) is two launches
  * on one stream:
  *
- *   k_plan  (1 work-group)   lays every running instance's chunks end to end
- *                            and resets the chunk ticket;
  *   k_pass  (persistent)     work-groups take chunks by ticket.  A chunk is a
  *                            contiguous run of the instance's agents in
  *                            reference index order; each agent is read once,
  *                            killed and updated (kernel/killAgents.cl:24-148,
  *                            kernel/updateAgents.cl:39-270) in registers,
- *                            ranked by its new state with 64-lane ballots and
- *                            an LDS scan, and written once into the chunk's
- *                            output segment for that state.  No work-group
- *                            waits on another;
+ *                            ranked by (input state, stayed / moved on) with
+ *                            64-lane ballots and an LDS scan, and written
+ *                            once into its input block's slots in the other
+ *                            buffer: stayers from the head, movers from the
+ *                            tail.  No work-group waits on another;
  *   k_prep  (1 group/instance) folds the chunks' counts and sums into the new
  *                            segment list -- the compaction of preScan + scan
  *                            + oldToNewAgents (kernel/preScan.cl,
@@ -22,12 +21,16 @@
  *                            computes the next ranges, printStats row and
  *                            calcStats sums (kernel/calcL1de.cl,
  *                            kernel/calcGender.cl), the larval mortality
- *                            table, and cuts the next chunks.
+ *                            table, and cuts the next chunks; its last
+ *                            work-group to finish lays every running
+ *                            instance's chunks end to end for the next
+ *                            pass (plan_all).
  *
- * Survivors keep reference order: chunks are in old-index order, and an
- * output segment (chunk c, state s) holds c's survivors of state s in
- * old-index order; the new state-s range is those segments for c = 0, 1, ...
- * which is the stable rank the reference's scans compute.  New eggs are not
+ * Survivors keep reference order: chunks are in old-index order, each run
+ * (chunk c, input state, stayed / moved on) holds its agents in old-index
+ * order, and the new state-s range is, chunk by chunk, the run moved in from
+ * the previous state, the run that stayed (and for BMS the run moved in from
+ * gravid) -- the stable rank the reference's scans compute.  New eggs are not
  * written (see sampo_model.h).
  */
 #include <hip/hip_runtime.h>
@@ -123,6 +126,7 @@ struct Ctx {
 	uint32_t ticket;
 	uint32_t total;
 	unsigned long long agent_steps;
+	uint32_t prep_done;    /* k_prep work-groups finished in the current launch */
 };
 
 /* ------------------------------------------------------------ the agent */
@@ -352,29 +356,51 @@ __device__ static inline uint32_t seg_find(const uint32_t *k0s, uint32_t lo, uin
 	return lo;
 }
 
-/* ------------------------------------------------------------ k_plan */
+/* ------------------------------------------------------------ plan */
 
-__global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par,
-                                               uint32_t ninst)
+/* Lay every running instance's chunks end to end for the next pass:
+ * pl_base[i] = first ticket of instance i, ctx->total = all chunks; reset
+ * the ticket.  Run by one work-group of 1024 threads: the last k_prep
+ * work-group to finish, or k_plan after an instance is released.  Other
+ * work-groups wrote the instances, so their fields are read with atomic
+ * loads at agent scope. */
+__device__ static void plan_all(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst)
 {
 	__shared__ uint32_t s_scan[1024];
-	__shared__ unsigned long long s_steps[1024];
 	const uint32_t tid = threadIdx.x;
+	auto chunks_of = [&](uint32_t i) -> uint32_t {
+		Inst *I = inst + i;
+		const uint32_t status = __hip_atomic_load(&I->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const uint32_t par = __hip_atomic_load(&I->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u;
+		const uint32_t active = __hip_atomic_load(&I->st[par].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		pl_par[i] = (uint8_t)par;
+		return status == SAMPO_RUNNING && active
+		           ? __hip_atomic_load(&I->st[par].nchunk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+		           : 0u;
+	};
+	if (ninst <= 64u) { /* one wave, no barriers */
+		if (tid < 64u) {
+			const uint32_t nc = tid < ninst ? chunks_of(tid) : 0u;
+			uint32_t x = nc;
+			for (int o = 1; o < 64; o <<= 1) {
+				uint32_t y = __shfl_up(x, o, 64);
+				if (tid >= (uint32_t)o) x += y;
+			}
+			if (tid < ninst) pl_base[tid] = x - nc;
+			const uint32_t all = __shfl(x, 63, 64);
+			if (tid == 0) {
+				pl_base[ninst] = all;
+				ctx->total = all;
+				ctx->ticket = 0u;
+			}
+		}
+		return;
+	}
 	uint32_t carry = 0;
-	unsigned long long steps = 0;
 	for (uint32_t b = 0; b < ninst; b += 1024u) {
 		uint32_t i = b + tid;
 		uint32_t nc = 0;
-		if (i < ninst) {
-			const Inst &I = inst[i];
-			uint32_t par = I.step & 1u;
-			const IterState &ip = I.st[par];
-			if (I.status == SAMPO_RUNNING && ip.active) {
-				nc = ip.nchunk;
-				steps += ip.n;
-			}
-			pl_par[i] = (uint8_t)par;
-		}
+		if (i < ninst) nc = chunks_of(i);
 		s_scan[tid] = nc;
 		__syncthreads();
 		for (uint32_t off = 1; off < 1024u; off <<= 1) {
@@ -387,45 +413,22 @@ __global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *p
 		carry += s_scan[1023];
 		__syncthreads();
 	}
-	s_steps[tid] = steps;
-	__syncthreads();
-	for (uint32_t off = 512; off > 0; off >>= 1) {
-		if (tid < off) s_steps[tid] += s_steps[tid + off];
-		__syncthreads();
-	}
 	if (tid == 0) {
 		pl_base[ninst] = carry;
 		ctx->total = carry;
 		ctx->ticket = 0u;
-		ctx->agent_steps += s_steps[0];
 	}
+}
+
+__global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par,
+                                               uint32_t ninst)
+{
+	plan_all(inst, ctx, pl_base, pl_par, ninst);
 }
 
 /* ------------------------------------------------------------ k_prep */
 
 #define PREP_WG 1024
-
-/* exclusive scan of v over the work-group; returns the total in *total */
-__device__ static inline uint32_t block_excl(uint32_t v, uint32_t *lds, uint32_t *total)
-{
-	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-	uint32_t x = v;
-	for (int o = 1; o < 64; o <<= 1) {
-		uint32_t y = __shfl_up(x, o, 64);
-		if (lane >= (uint32_t)o) x += y;
-	}
-	if (lane == 63u) lds[w] = x;
-	__syncthreads();
-	uint32_t before = 0, all = 0;
-	for (uint32_t q = 0; q < PREP_WG / 64; ++q) {
-		if (q < w) before += lds[q];
-		all += lds[q];
-	}
-	__syncthreads();
-	*total = all;
-	return before + x - v;
-}
-
 
 /* chunk size of an iteration with N agents: at least chunk_size, and at
  * most target_chunks chunks */
@@ -440,34 +443,65 @@ __device__ static inline uint32_t chunk_agents(const Inst *I, uint32_t N)
 	return cs;
 }
 
-/* Output runs of one chunk in the next iteration's dense order: for each new
- * state s', the run moved in from the previous state, the run that stayed,
- * and for BMS the run moved in from gravid -- 16 runs per chunk.  Run q of
- * the list (new state, chunk, run) over C chunks: its new state, chunk and
- * source (input state, moved). */
-__device__ static inline void run_of(uint32_t q, uint32_t C, uint32_t *sn_out, uint32_t *c_out, uint32_t *src,
-                                     uint32_t *moved)
+/* Exclusive scan of N values per thread over the work-group, all at once;
+ * the totals land in tot (shared). */
+template <int N>
+__device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot)
 {
-	/* first run of new state s' within a chunk: 0, 1, 3, 5, 7, 9, 12, 14 (16 in all) */
-	const uint32_t sn = (q >= C) + (q >= 3u * C) + (q >= 5u * C) + (q >= 7u * C) + (q >= 9u * C) + (q >= 12u * C) +
-	                    (q >= 14u * C);
-	const uint32_t first = sn == 0u ? 0u : (sn <= 5u ? 2u * sn - 1u : 2u * sn);
-	const uint32_t nk = sn == 0u ? 1u : (sn == (uint32_t)S_BMS ? 3u : 2u);
-	const uint32_t rel = q - C * first, c = rel / nk, kind = rel - c * nk;
-	*sn_out = sn;
-	*c_out = c;
-	if (sn == 0u) { *src = 0u; *moved = 0u; }
-	else if (kind == 0u) { *src = sn - 1u; *moved = 1u; }
-	else if (kind == 1u) { *src = sn; *moved = 0u; }
-	else { *src = S_GRAVID; *moved = 1u; }
+	__shared__ uint32_t lds[PREP_WG / 64][N];
+	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+	uint32_t x[N];
+#pragma unroll
+	for (int i = 0; i < N; ++i) x[i] = v[i];
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+		for (int i = 0; i < N; ++i) {
+			uint32_t y = __shfl_up(x[i], o, 64);
+			if (lane >= (uint32_t)o) x[i] += y;
+		}
+	}
+	if (lane == 63u)
+#pragma unroll
+		for (int i = 0; i < N; ++i) lds[w][i] = x[i];
+	__syncthreads();
+	if (tid < (uint32_t)N) { /* wave totals -> exclusive wave offsets, per column */
+		uint32_t run = 0;
+		for (uint32_t q = 0; q < PREP_WG / 64; ++q) {
+			const uint32_t t = lds[q][tid];
+			lds[q][tid] = run;
+			run += t;
+		}
+		tot[tid] = run;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < N; ++i) ex[i] = lds[w][i] + x[i] - v[i];
+	__syncthreads();
+}
+
+/* The runs of one chunk in the next iteration's dense order: for each new
+ * state s', the run moved in from the previous state, the run that stayed,
+ * and for BMS the run moved in from gravid.  Their counter rows in cout (q
+ * stayed in input state q, 8 + q moved on from input state q): */
+__device__ static inline uint32_t runs_of(uint32_t sn) { return sn == 0u ? 1u : (sn == (uint32_t)S_BMS ? 3u : 2u); }
+__device__ static inline uint32_t run_row(uint32_t sn, uint32_t kind)
+{
+	return sn == 0u ? 0u : (kind == 0u ? 8u + sn - 1u : (kind == 1u ? sn : 8u + S_GRAVID));
 }
 
 /* Stage A: fold the finished iteration of I (parity par) into the segment
- * list, ranges and sums of the next one (parity par ^ 1). */
+ * list, ranges and sums of the next one (parity par ^ 1).  Thread t owns
+ * chunks [tK, tK + K): it loads their 16 run counts and 8 sums (consecutive
+ * lanes, consecutive chunks), one scan over the work-group gives each
+ * thread's offset in every new state's range, and each thread writes its own
+ * chunks' segments.  Only non-empty runs get a segment, so a tile of the
+ * pass never overlaps more segments than it has agents (plus one); the scan
+ * also counts each thread's non-empty runs per new state. */
 __device__ static void fold_iteration(Inst *I, uint32_t par)
 {
-	__shared__ uint32_t lds[PREP_WG / 64];
-	__shared__ uint32_t s_tot[NOUT], s_kst[9], s_vst[8], s_cs, s_C;
+	constexpr uint32_t KR = 2; /* chunks per thread held in registers */
+	__shared__ uint32_t s_sum[8], s_kst[9], s_vst[8], s_sb[8], s_tot[16], s_cs, s_C;
 	const uint32_t tid = threadIdx.x, lane = tid & 63u;
 	const IterState &ip = I->st[par];
 	IterState &np = I->st[par ^ 1u];
@@ -475,35 +509,64 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	const uint32_t *co = I->cout;
 	Chunk *nch = I->chunk[par ^ 1u];
 	Seg *seg = I->seg[par ^ 1u];
-	if (tid < NOUT) s_tot[tid] = 0u;
-	__syncthreads();
-	/* totals: each thread sums chunks tid, tid + PREP_WG, ... of every
-	 * counter row; loads are issued in batches so each batch waits once */
-	{
-		uint32_t v[NOUT];
-		for (int q = 0; q < NOUT; ++q) v[q] = 0;
-		for (uint32_t c = tid; c < C; c += PREP_WG) {
-			uint32_t x[NOUT];
+	if (tid < 8u) s_sum[tid] = 0u;
+	const uint32_t K = (C + PREP_WG - 1u) / PREP_WG;
+	const uint32_t c_lo = tid * K < C ? tid * K : C, c_hi = c_lo + K < C ? c_lo + K : C;
+	const bool in_regs = K <= KR;
+	auto count = [&](uint32_t row, uint32_t c) -> uint32_t { return co[(size_t)row * cmax + c]; };
+	/* loads: the run counts of this thread's chunks (when few) and their sums */
+	uint32_t nr[KR][16];
+	uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	if (in_regs) {
 #pragma unroll
-			for (int q = 0; q < NOUT; ++q) x[q] = __builtin_nontemporal_load(co + (size_t)q * cmax + c);
+		for (uint32_t j = 0; j < KR; ++j)
 #pragma unroll
-			for (int q = 0; q < NOUT; ++q) v[q] += x[q];
-		}
-#pragma unroll
-		for (int q = 0; q < NOUT; ++q) {
-			uint32_t r = wave_sum(v[q]);
-			if (lane == 0 && r) atomicAdd(&s_tot[q], r);
-		}
+			for (uint32_t r = 0; r < 16u; ++r) nr[j][r] = c_lo + j < c_hi ? count(r, c_lo + j) : 0u;
 	}
-	__syncthreads();
-	const uint32_t E = s_tot[CO_SUMS];
+	for (uint32_t c = c_lo; c < c_hi; ++c) {
+#pragma unroll
+		for (int q = 0; q < 8; ++q) v[q] += __builtin_nontemporal_load(co + (size_t)(CO_SUMS + q) * cmax + c);
+	}
+	/* per new state: agents (w) and non-empty runs (ne) of this thread's chunks */
+	uint32_t wv[16];
+#pragma unroll
+	for (int i = 0; i < 16; ++i) wv[i] = 0u;
+	if (in_regs) {
+#pragma unroll
+		for (uint32_t j = 0; j < KR; ++j)
+#pragma unroll
+			for (uint32_t sn = 0; sn < 8u; ++sn)
+#pragma unroll
+				for (uint32_t kind = 0; kind < 3u; ++kind) {
+					if (kind >= runs_of(sn)) continue;
+					const uint32_t n = nr[j][run_row(sn, kind)]; /* zero past c_hi */
+					wv[sn] += n;
+					wv[8u + sn] += n ? 1u : 0u;
+				}
+	} else {
+		for (uint32_t c = c_lo; c < c_hi; ++c)
+#pragma unroll
+			for (uint32_t sn = 0; sn < 8u; ++sn)
+#pragma unroll
+				for (uint32_t kind = 0; kind < 3u; ++kind) {
+					if (kind >= runs_of(sn)) continue;
+					const uint32_t n = count(run_row(sn, kind), c);
+					wv[sn] += n;
+					wv[8u + sn] += n ? 1u : 0u;
+				}
+	}
+#pragma unroll
+	for (int q = 0; q < 8; ++q) {
+		uint32_t r = wave_sum(v[q]);
+		if (lane == 0 && r) atomicAdd(&s_sum[q], r);
+	}
+	uint32_t ex[16];
+	block_excl_n<16>(wv, ex, s_tot);
+	const uint32_t E = s_sum[0];
 	if (tid == 0) {
-		uint32_t k = 0, vend = 0;
+		uint32_t k = 0, vend = 0, sb = E > 0u ? 1u : 0u;
 		for (uint32_t s = 0; s < 8u; ++s) {
-			uint32_t c = s_tot[s];
-			if (s == 0u) c += E;
-			else c += s_tot[8u + s - 1u];
-			if (s == S_BMS) c += s_tot[8u + S_GRAVID];
+			const uint32_t c = s_tot[s] + (s == 0u ? E : 0u);
 			np.cnt[s] = c;
 			s_kst[s] = k;
 			k += c;
@@ -511,18 +574,21 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 			np.vstart[s] = vs;
 			s_vst[s] = vs;
 			vend = vs + c;
+			s_sb[s] = sb;
+			sb += s_tot[8 + s];
 		}
 		s_kst[8] = k;
+		np.nseg = sb;
 		np.enew = E;
 		np.cur = ip.cur ^ 1u;
 		np.step = ip.step + 1u;
-		I->bites = s_tot[CO_SUMS + 1];
-		I->ibites = s_tot[CO_SUMS + 2];
-		I->ncyc = s_tot[CO_SUMS + 3];
-		I->scyc = s_tot[CO_SUMS + 4];
-		I->l1de = s_tot[CO_SUMS + 5];
-		I->females = s_tot[CO_SUMS + 6];
-		I->potinf = s_tot[CO_SUMS + 7];
+		I->bites = s_sum[1];
+		I->ibites = s_sum[2];
+		I->ncyc = s_sum[3];
+		I->scyc = s_sum[4];
+		I->l1de = s_sum[5];
+		I->females = s_sum[6];
+		I->potinf = s_sum[7];
 		const uint32_t cs = chunk_agents(I, k);
 		uint32_t nc = (uint32_t)(((uint64_t)k + cs - 1u) / cs);
 		s_cs = cs;
@@ -533,78 +599,66 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	__syncthreads();
 	const uint32_t cs = s_cs, NC = s_C;
 	const uint32_t ocs = ip.cs, on = ip.n; /* chunking of the finished iteration */
-	/* segments: newborns, then the runs in (new state, chunk, run) order;
-	 * each thread takes a contiguous stretch of runs.  Each new chunk's first
-	 * segment is recorded as its start is passed. */
-	const uint32_t head = E > 0u ? 1u : 0u;
-	if (head) {
+	/* segments: the newborns, then per new state the non-empty runs of
+	 * chunks 0, 1, ...  Each new chunk's first segment is recorded as its
+	 * start is passed. */
+	if (E > 0u) {
 		if (tid == 0) {
 			Seg sg = {0u, 0u, PB_NEWBORN, 0u};
 			seg[0] = sg;
 		}
 		for (uint32_t c = tid; c < NC && c * cs < E; c += PREP_WG) nch[c].seg0 = 0u;
 	}
-	const uint32_t total = 16u * C;
-	const uint32_t L = (total + PREP_WG - 1u) / PREP_WG;
-	const uint32_t q0 = tid * L < total ? tid * L : total;
-	const uint32_t q1 = q0 + L < total ? q0 + L : total;
-	constexpr uint32_t B = 8; /* loads in flight per batch */
-	uint32_t sk = 0, si = 0;
-	for (uint32_t qb = q0; qb < q1; qb += B) {
-		uint32_t n[B];
+	if (c_lo >= c_hi) return;
+	/* one run: its segment at dense k0, index idx, and the chunk starts it covers */
+	auto emit = [&](uint32_t sn, uint32_t row, uint32_t c, uint32_t n, uint32_t &k0, uint32_t &idx, uint32_t &nxc,
+	                uint32_t &nxk) {
+		const uint32_t src = row & 7u, mv = row >> 3;
+		/* the source block [max(kstart[src], c*ocs), min(kstart[src+1], (c+1)*ocs, n)) */
+		const uint32_t ck0 = c * ocs, ck1 = ck0 + ocs < on ? ck0 + ocs : on;
+		const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
+		const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
+		Seg sg;
+		sg.k0 = k0;
+		sg.state = sn | (mv ? SEG_REV : 0u);
+		sg.pbase = mv ? be - 1u : bh;
+		sg.vbase = s_vst[sn] - s_kst[sn] + k0;
+		seg[idx] = sg;
+		for (; nxc < NC && nxk < k0 + n; ++nxc, nxk += cs) nch[nxc].seg0 = idx;
+		++idx;
+		k0 += n;
+	};
+	if (in_regs) {
 #pragma unroll
-		for (uint32_t u = 0; u < B; ++u) {
-			uint32_t q = qb + u < q1 ? qb + u : q1 - 1u;
-			uint32_t sn, c, src, mv;
-			run_of(q, C, &sn, &c, &src, &mv);
-			n[u] = co[(size_t)(mv * 8u + src) * cmax + c];
+		for (uint32_t sn = 0; sn < 8u; ++sn) {
+			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
+			uint32_t idx = s_sb[sn] + ex[8u + sn];
+			uint32_t nxc = (k0 + cs - 1u) / cs, nxk = nxc * cs; /* next new chunk start at or after k0 */
+#pragma unroll
+			for (uint32_t j = 0; j < KR; ++j) {
+				if (c_lo + j >= c_hi) continue;
+#pragma unroll
+				for (uint32_t kind = 0; kind < 3u; ++kind) {
+					if (kind >= runs_of(sn)) continue;
+					const uint32_t row = run_row(sn, kind);
+					const uint32_t n = nr[j][row];
+					if (n) emit(sn, row, c_lo + j, n, k0, idx, nxc, nxk);
+				}
+			}
 		}
-#pragma unroll
-		for (uint32_t u = 0; u < B; ++u)
-			if (qb + u >= q1) n[u] = 0u;
-#pragma unroll
-		for (uint32_t u = 0; u < B; ++u) {
-			sk += n[u];
-			si += n[u] ? 1u : 0u;
-		}
-	}
-	uint32_t tk, ti;
-	uint32_t k0 = E + block_excl(sk, lds, &tk);
-	uint32_t idx = head + block_excl(si, lds, &ti);
-	for (uint32_t qb = q0; qb < q1; qb += B) {
-		uint32_t n[B], pb[B], stw[B];
-#pragma unroll
-		for (uint32_t u = 0; u < B; ++u) { /* unconditional loads: one wait per batch */
-			uint32_t q = qb + u < q1 ? qb + u : q1 - 1u;
-			uint32_t sn, c, src, mv;
-			run_of(q, C, &sn, &c, &src, &mv);
-			n[u] = co[(size_t)(mv * 8u + src) * cmax + c];
-			/* the source block [max(kstart[src], c*ocs), min(kstart[src+1], (c+1)*ocs, n)) */
-			const uint32_t ck0 = c * ocs, ck1 = ck0 + ocs < on ? ck0 + ocs : on;
-			const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
-			const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
-			pb[u] = mv ? be - 1u : bh;
-			stw[u] = sn | (mv ? SEG_REV : 0u);
-		}
-#pragma unroll
-		for (uint32_t u = 0; u < B; ++u)
-			if (qb + u >= q1) n[u] = 0u;
-#pragma unroll
-		for (uint32_t u = 0; u < B; ++u) {
-			if (!n[u]) continue;
-			const uint32_t sn = stw[u] & 0xffu;
-			Seg sg;
-			sg.k0 = k0;
-			sg.state = stw[u];
-			sg.pbase = pb[u];
-			sg.vbase = s_vst[sn] + (k0 - s_kst[sn]);
-			seg[idx] = sg;
-			for (uint32_t nc = (k0 + cs - 1u) / cs; nc < NC && nc * cs < k0 + n[u]; ++nc) nch[nc].seg0 = idx;
-			++idx;
-			k0 += n[u];
+	} else {
+		for (uint32_t sn = 0; sn < 8u; ++sn) {
+			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
+			uint32_t idx = s_sb[sn] + ex[8u + sn];
+			uint32_t nxc = (k0 + cs - 1u) / cs, nxk = nxc * cs;
+			for (uint32_t c = c_lo; c < c_hi; ++c)
+				for (uint32_t kind = 0; kind < runs_of(sn); ++kind) {
+					const uint32_t row = run_row(sn, kind);
+					const uint32_t n = count(row, c);
+					if (n) emit(sn, row, c, n, k0, idx, nxc, nxk);
+				}
 		}
 	}
-	if (tid == 0) np.nseg = head + ti;
 }
 
 /* Stage B: calcStats of the iteration in st[par] (src/abms.c:230-310), its
@@ -711,10 +765,10 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 
 /* The instance record is worked on in LDS: the bookkeeping is a chain of
  * dependent scalar updates that would otherwise each wait on global memory. */
-__global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts)
+__global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par)
 {
 	__shared__ Inst sI;
-	__shared__ uint32_t s_fold, s_begin, s_par;
+	__shared__ uint32_t s_fold, s_begin, s_par, s_last;
 	Inst *G = insts + blockIdx.x;
 	{
 		const uint32_t *src = (const uint32_t *)G;
@@ -727,29 +781,46 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts)
 		s_fold = I->status == SAMPO_RUNNING && I->pending;
 		s_par = I->step & 1u;
 		s_begin = I->needs_begin;
+		/* the iteration being folded was run over this many agents */
+		if (s_fold) atomicAdd(&ctx->agent_steps, (unsigned long long)I->st[I->step & 1u].n);
 	}
 	__syncthreads();
-	if (!s_fold && !s_begin) return;
-	if (s_fold) {
-		fold_iteration(I, s_par);
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			I->pending = 0u;
-			I->step = I->step + 1u;
-			s_begin = 1u;
-			s_par ^= 1u;
+	if (s_fold || s_begin) {
+		if (s_fold) {
+			fold_iteration(I, s_par);
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				I->pending = 0u;
+				I->step = I->step + 1u;
+				s_begin = 1u;
+				s_par ^= 1u;
+			}
 		}
-	}
-	__syncthreads();
-	if (s_begin) {
-		begin_iteration(I, s_par);
-		if (threadIdx.x == 0) I->needs_begin = 0u;
-	}
-	__syncthreads();
-	{
+		__syncthreads();
+		if (s_begin) {
+			begin_iteration(I, s_par);
+			if (threadIdx.x == 0) I->needs_begin = 0u;
+		}
+		__syncthreads();
 		const uint32_t *src = (const uint32_t *)&sI;
 		uint32_t *dst = (uint32_t *)G;
 		for (uint32_t q = threadIdx.x; q < sizeof(Inst) / 4u; q += PREP_WG) dst[q] = src[q];
+	}
+	/* the last work-group to finish plans the next pass */
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		if (gridDim.x == 1u) {
+			s_last = 1u; /* alone: nothing to publish to other work-groups */
+		} else {
+			__threadfence();
+			s_last = atomicAdd(&ctx->prep_done, 1u) == gridDim.x - 1u;
+		}
+	}
+	__syncthreads();
+	if (s_last) {
+		if (gridDim.x > 1u) __threadfence();
+		plan_all(insts, ctx, pl_base, pl_par, gridDim.x);
+		if (threadIdx.x == 0) ctx->prep_done = 0u;
 	}
 }
 
@@ -761,6 +832,22 @@ typedef __attribute__((address_space(1))) void glob_void;
  * them are global (not flat) instructions, which the compiler knows cannot
  * touch LDS -- so they never make it wait before an LDS access. */
 typedef __attribute__((address_space(1))) uint32_t gu32;
+
+/* A pointer every lane holds the same value of, moved to scalar registers,
+ * so that addressing is scalar base + 32-bit lane offset. */
+__device__ static inline gu32 *uniform_ptr(const uint32_t *p)
+{
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return (gu32 *)(((uint64_t)hi << 32) | lo);
+}
+
+/* &base[slot] for slot < 2^30 (P is capped there): a 32-bit byte offset */
+__device__ static inline gu32 *at_slot(gu32 *base, uint32_t slot)
+{
+	return (gu32 *)((__attribute__((address_space(1))) char *)base + (uint32_t)(slot << 2));
+}
 
 /* Barrier for LDS traffic only.  The pass never exchanges global memory
  * between the waves of a work-group, and the direct-to-LDS loads of a tile
@@ -810,6 +897,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
 	const uint32_t total = ctx->total;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+	uint32_t next = 0; /* global index of the segment holding the last issued tile's end */
 
 	/* stage the segments from global index `from` on until the window covers
 	 * dense index te - 1 (wave 0) */
@@ -858,32 +946,45 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 		__builtin_amdgcn_s_waitcnt(0x0F70);
 	};
 
-	/* sources of tile [ts, te) from the window, and their loads into buf;
-	 * returns the global index of the segment holding te - 1 */
-	auto issue = [&](uint32_t ts, uint32_t te, uint32_t (&buf)[NFIELD][TILE]) -> uint32_t {
+	/* Sources of tile [ts, te) from the window and their loads into buf.
+	 * `next` enters as the global index of a segment starting at or before
+	 * ts (the one holding the previous tile's last agent, where the window
+	 * also starts after a restage) and leaves as the one holding te - 1.
+	 * The tile's segments are found by a uniform forward walk; each lane then
+	 * picks its own among those few.  Returns the tile's input-state span. */
+	uint32_t span_lo = 0, span_hi = 0;
+	auto issue = [&](uint32_t ts, uint32_t te, uint32_t (&buf)[NFIELD][TILE], gu32 *const (&fb)[NFIELD]) {
 		const uint32_t m = s_wm;
+		uint32_t q = next - s_wlo;
+		while (q + 1u < m && s_k0[q + 1u] <= ts) ++q;
+		const uint32_t q_lo = q;
+		while (q + 1u < m && s_k0[q + 1u] < te) ++q;
+		const uint32_t q_hi = q;
+		span_lo = s_st[q_lo] & 0xffu;
+		span_hi = s_st[q_hi] & 0xffu;
+		next = s_wlo + q_hi;
 #pragma unroll
 		for (int jx = 0; jx < APT; ++jx) {
 			const uint32_t k = ts + (uint32_t)jx * WG + tid;
 			const uint32_t o = (uint32_t)jx * WG + tid;
 			s_ss[o] = S_DEAD;
 			if (k < te) {
-				const uint32_t q = seg_find(s_k0, 0u, m, k);
-				const uint32_t r = k - s_k0[q];
-				const bool born = s_pb[q] == PB_NEWBORN;
-				s_ss[o] = (s_st[q] & 0xffu) | (born ? 0x100u : 0u);
-				s_sg[o] = s_vb[q] + r;
+				uint32_t qq = q_lo;
+				for (uint32_t j = q_lo + 1u; j <= q_hi; ++j) qq += k >= s_k0[j] ? 1u : 0u;
+				const uint32_t r = k - s_k0[qq];
+				const uint32_t pb = s_pb[qq], st = s_st[qq];
+				const bool born = pb == PB_NEWBORN;
+				s_ss[o] = (st & 0xffu) | (born ? 0x100u : 0u);
+				s_sg[o] = s_vb[qq] + r;
 				if (!born) {
-					const uint32_t p = seg_slot(s_pb[q], s_st[q], r);
-					const gu32 *b = (const gu32 *)T.bin + p;
+					const uint32_t p = seg_slot(pb, st, r);
 #pragma unroll
 					for (int f = 0; f < NFIELD; ++f)
-						__builtin_amdgcn_global_load_lds((glob_void *)(b + (size_t)f * T.P),
+						__builtin_amdgcn_global_load_lds((glob_void *)at_slot(fb[f], p),
 						                                 (lds_void *)&buf[f][(uint32_t)jx * WG + w * 64u], 4, 0, 0);
 				}
 			}
 		}
-		return s_wlo + seg_find(s_k0, 0u, m, te - 1u);
 	};
 
 	for (;;) {
@@ -936,18 +1037,29 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			s_head[tid] = h;
 			s_end[tid] = z > h ? z : h;
 		}
-		const size_t P = T.P;
 		const uint32_t k0 = T.ch.k0, k1 = T.ch.k1;
 		Acc acc = {0, 0, 0, 0, 0};
 		uint32_t l1de = 0, fem = 0, pinf = 0;
 
+		/* per-field bases of both buffers in scalar registers */
+		gu32 *fin[NFIELD], *fout[NFIELD];
+		{
+			gu32 *bi = uniform_ptr(T.bin), *bo = uniform_ptr(T.bout);
+			const uint32_t Pu = __builtin_amdgcn_readfirstlane(T.P);
+#pragma unroll
+			for (int f = 0; f < NFIELD; ++f) {
+				fin[f] = bi + (size_t)f * Pu;
+				fout[f] = bo + (size_t)f * Pu;
+			}
+		}
 		/* prologue: tile 0's window and loads */
+		next = T.ch.seg0;
 		{
 			const uint32_t te0 = k0 + TILE < k1 ? k0 + TILE : k1;
-			if (w == 0) stage(T.ch.seg0, te0);
+			if (w == 0) stage(next, te0);
 			lds_barrier();
 		}
-		uint32_t next = issue(k0, k0 + TILE < k1 ? k0 + TILE : k1, bufA);
+		issue(k0, k0 + TILE < k1 ? k0 + TILE : k1, bufA, fin);
 
 		/* one tile: its agents are in cur (loaded) or regenerated; the next
 		 * tile's loads go to nxt */
@@ -980,13 +1092,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					}
 				}
 			}
-			/* the input states the tile spans (dense order is state-sorted) */
-			uint32_t s_lo, s_hi;
-			{
-				const uint32_t m = s_wm;
-				s_lo = s_st[seg_find(s_k0, 0u, m, ts)] & 0xffu;
-				s_hi = s_st[seg_find(s_k0, 0u, m, te - 1u)] & 0xffu;
-			}
+			/* the input states the tile spans (dense order is state-sorted),
+			 * found when its loads were issued */
+			const uint32_t s_lo = __builtin_amdgcn_readfirstlane(span_lo);
+			const uint32_t s_hi = __builtin_amdgcn_readfirstlane(span_hi);
 			/* next tile: window (wave 0), barrier, loads into nxt */
 			const uint32_t ts2 = te, te2 = ts2 + TILE < k1 ? ts2 + TILE : k1;
 			const bool more = ts2 < k1;
@@ -994,7 +1103,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			if (more) {
 				if (w == 0 && te2 > s_wend) stage(next, te2);
 				lds_barrier();
-				next = issue(ts2, te2, nxt);
+				issue(ts2, te2, nxt, fin);
 			}
 			uint32_t ns[APT], lr[APT];
 #pragma unroll 1
@@ -1053,15 +1162,8 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t o = (uint32_t)jx * WG + tid;
 					const uint32_t off = s_wc[jx][w][cl] + lr[jx];
 					const uint32_t p = (cl & 1u) ? s_end[cl >> 1] - 1u - off : s_head[cl >> 1] + off;
-					gu32 *b = (gu32 *)T.bout;
-					b[p] = cur[F_AGE][o];
-					b[P + p] = cur[F_HIS][o];
-					b[2 * P + p] = cur[F_DLY][o];
-					b[3 * P + p] = cur[F_AUX][o];
-					b[4 * P + p] = cur[F_CSD][o];
-					b[5 * P + p] = cur[F_AVAIL][o];
-					b[6 * P + p] = cur[F_CYC][o];
-					b[7 * P + p] = cur[F_MISC][o];
+#pragma unroll
+					for (int f = 0; f < NFIELD; ++f) *at_slot(fout[f], p) = cur[f][o];
 				}
 			}
 			/* the next tile rewrites s_wc only after its first barrier */
@@ -1431,7 +1533,8 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	if (sc->mode != SAMPO_MODE_REFERENCE && sc->mode != SAMPO_MODE_CONFIG)
 		return fail(SAMPO_E_ARG, "unknown mode%s%ld");
 	if (sc->max_agents == 0) return fail(SAMPO_E_ARG, "max_agents is required%s%ld");
-	if (round256(sc->max_agents) > (1ull << 31)) return fail(SAMPO_E_ARG, "max_agents exceeds 2^31%s%ld");
+	/* slots are addressed with 32-bit byte offsets per field */
+	if (round256(sc->max_agents) > (1ull << 30)) return fail(SAMPO_E_ARG, "max_agents exceeds 2^30%s%ld");
 	uint32_t id = e->max_slots;
 	for (uint32_t i = 0; i < e->max_slots; ++i)
 		if (!e->slots[i].used) { id = i; break; }
@@ -1540,7 +1643,8 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		return fail(SAMPO_E_HIP, "upload failed: %s%ld", hipGetErrorString(err));
 	}
 	/* begin iteration 0 on the device (calcStats of src/abms.c:503) */
-	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst);
+	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
+		                   e->d_pl_base, e->d_pl_par);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(e->stream));
 	S.used = true;
@@ -1556,6 +1660,11 @@ extern "C" int sampo_engine_release(sampo_engine *e, uint32_t slot)
 	HIPCHK(hipStreamSynchronize(e->stream));
 	uint32_t st = SAMPO_EMPTY;
 	HIPCHK(hipMemcpy(&e->d_inst[slot].status, &st, sizeof(st), hipMemcpyHostToDevice));
+	/* plan the next pass without it before its memory goes */
+	hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
+	                   e->max_slots);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipStreamSynchronize(e->stream));
 	free_slot(e->slots[slot]);
 	return 0;
 }
@@ -1632,7 +1741,8 @@ extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *g
 		++count;
 	}
 	if (resumed) {
-		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst);
+		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
+		                   e->d_pl_base, e->d_pl_par);
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipStreamSynchronize(e->stream));
 	}
@@ -1640,15 +1750,16 @@ extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *g
 	return rc;
 }
 
+/* One iteration: the pass over every running instance, then k_prep, which
+ * folds it, begins the next iteration and plans the next pass. */
 static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after_pass)
 {
-	hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
-	                   e->max_slots);
 	if (before_pass) HIPCHK(hipEventRecord(before_pass, e->stream));
 	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots};
 	HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
 	if (after_pass) HIPCHK(hipEventRecord(after_pass, e->stream));
-	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst);
+	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
+	                   e->d_pl_par);
 	return 0;
 }
 
