@@ -340,6 +340,11 @@ def icl_lib():
             raise SampoError(f"{ICL_LIB} is not built; run python -m sampo_amd.build")
         L = C.CDLL(ICL_LIB)
         L.sampo_abms_run.argtypes = [C.POINTER(AbmsParams), C.POINTER(AbmsResult)]
+        L.sampo_format_stats.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32, C.c_float, C.c_float, C.POINTER(Row)]
+        L.sampo_format_population.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(Layout)]
+        L.sampo_format_histogram.argtypes = [C.c_char_p, C.c_size_t, C.c_void_p]
+        L.sampo_read_single_val_file.argtypes = [C.c_char_p, C.c_uint32, C.c_void_p]
+        L.sampo_read_environment.argtypes = [C.c_char_p, C.c_uint32, C.c_float, C.c_void_p]
         _icl = L
     return _icl
 
@@ -385,3 +390,59 @@ def run_abms(*, seed, n_eggs=0, max_agents, steps, temp, env, serial_scan=False,
         span = int(layout[7][1])
         out.update(agents=ag[:span], ages=ge[:span], states=st[:span])
     return out
+
+
+# ---------------------------------------------------------------- text input/output
+
+def format_stats(step, temp, row, hours_per_step=1.0):
+    """printStats line (src/abms.c:67-81) of one stats row (ROW_DTYPE record)."""
+    r = Row()
+    for s_ in range(8):
+        r.count[s_] = int(row["count"][s_])
+    for f in ("l1de", "females", "potInfective", "numCycles", "sumCycles", "bites", "infectBites", "valid"):
+        setattr(r, f, int(row[f]))
+    buf = C.create_string_buffer(512)
+    n = icl_lib().sampo_format_stats(buf, 512, step, hours_per_step, temp, C.byref(r))
+    if n < 0:
+        raise SampoError("format_stats failed")
+    return buf.raw[:n].decode()
+
+
+def format_population(layout):
+    """printPopulation text (src/abms.c:83-92) of an (8, 2) range array."""
+    lay = Layout()
+    for s_ in range(8):
+        lay.start[s_], lay.end[s_] = int(layout[s_][0]), int(layout[s_][1])
+    buf = C.create_string_buffer(1024)
+    n = icl_lib().sampo_format_population(buf, 1024, C.byref(lay))
+    if n < 0:
+        raise SampoError("format_population failed")
+    return buf.raw[:n].decode()
+
+
+def format_histogram(hist):
+    """histogram.txt content (src/abms.c:176-180)."""
+    h = np.ascontiguousarray(hist, dtype=np.uint32)
+    buf = C.create_string_buffer(2048)
+    n = icl_lib().sampo_format_histogram(buf, 2048, h.ctypes.data)
+    if n < 0:
+        raise SampoError("format_histogram failed")
+    return buf.raw[:n].decode()
+
+
+def read_single_val_file(path, n):
+    """readSingleValFile (src/abms.c:94-108): up to n floats."""
+    out = np.zeros(n, np.float32)
+    k = icl_lib().sampo_read_single_val_file(str(path).encode(), n, out.ctypes.data)
+    if k < 0:
+        raise SampoError(f"cannot read {path}")
+    return out[:k]
+
+
+def read_environment(path, n, carrying_capacity):
+    """readEnvironment (src/abms.c:110-137): up to n environments."""
+    out = np.zeros(n, dtype=[(nm, "<f4") for nm, _ in Env._fields_])
+    k = icl_lib().sampo_read_environment(str(path).encode(), n, carrying_capacity, out.ctypes.data)
+    if k < 0:
+        raise SampoError(f"cannot read {path}")
+    return out[:k]

@@ -2,8 +2,9 @@
 
 libsampo_engine.so  fused, batched agent-step engine (csrc/sampo_engine.hip)
 libsampo_icl.so     the reference's icl_* runtime API, scan API and kernel
-                    contract over HIP, plus the reference driver loop
-                    (csrc/sampo_icl.hip, csrc/sampo_abms.c)
+                    contract over HIP, plus the reference driver loop and its
+                    text input/output (csrc/sampo_icl.hip, csrc/sampo_abms.c,
+                    csrc/sampo_report.c)
 
 Every HIP source is compiled with -ffp-contract=off and IEEE division/sqrt so
 that device arithmetic follows the canonical numeric contract bit for bit,
@@ -27,7 +28,7 @@ HIP_FLAGS = [
 
 TARGETS = {
     "libsampo_engine.so": ["sampo_engine.hip"],
-    "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c"],
+    "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c", "sampo_report.c"],
 }
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 

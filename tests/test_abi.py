@@ -22,6 +22,7 @@ LIBS = {
     "sampo_engine.h": os.path.join(ROOT, "sampo_amd", "libsampo_engine.so"),
     "sampo_icl.h": os.path.join(ROOT, "sampo_amd", "libsampo_icl.so"),
     "sampo_abms.h": os.path.join(ROOT, "sampo_amd", "libsampo_icl.so"),
+    "sampo_report.h": os.path.join(ROOT, "sampo_amd", "libsampo_icl.so"),
 }
 CONST = json.load(open(os.path.join(ROOT, "tests", "golden", "constants.json")))
 
