@@ -30,6 +30,9 @@ typedef struct {
 	float dmr[4];
 	const sampo_layout *init_layout;
 	const sampo_agent *init_agents; /* dense in range order */
+	/* the speciesHeader argument (src/abms.c:565-568): NULL or a name ending in
+	 * properties.h for include/properties.h, else a species file path */
+	const char *species;
 } sampo_abms_params;
 
 typedef struct {

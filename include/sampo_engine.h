@@ -88,6 +88,51 @@ typedef struct {
 	uint32_t valid;        /* 1 if the reference would have printed this row */
 } sampo_row;
 
+/* Species behaviour: the runtime form of the reference's species plug-in,
+ * the header chosen with -DSPECIES=<header> when the kernels are built
+ * (src/abms.c:565-568, kernel/updateAgents.cl:6-23).  Every function of
+ * include/properties.h is one member of the parametric family below;
+ * sampo_species_default() gives include/properties.h itself.  Temperatures
+ * are T in degC; binary32 unless noted, evaluated without contraction. */
+typedef struct {
+	float egg_incubation[2];   /* eggsIncubation (properties.h:29-31): [0]*T + [1] */
+	float hatch_threshold[4];  /* eggsHatching (properties.h:39-60): first i with p < threshold[i], else 4; */
+	float hatch_slope[5];      /*   (float)(uint)(slope[i]*p + offset[i] + 0.5f) */
+	float hatch_offset[5];
+	float larva_delay[2];      /* larvaDelay (properties.h:87-91): BoxMuller(mean [0], sd [1]) */
+	float dev_temp_range[2];   /* larvaDevelopment (properties.h:105-109): T clamped to [lo, hi], */
+	float dev_rates[32];       /*   dev_rates[min((int)T - (int)lo, 31)] per hour (properties.h:94-98) */
+	double pupa_delay[2];      /* pupaDelay (properties.h:133-135): [0]*T + [1] in binary64, rounded once */
+	float immature_delay[2];   /* immatureDelay (properties.h:159-161): [0]*T + [1] */
+	float mate_seeking_delay;  /* mateSeekingDelay (properties.h:185-187) */
+	float bms_delay;           /* bloodMealSeekingDelay (properties.h:212-214) */
+	float bmd_delay[2];        /* bloodMealDigestingDelay (properties.h:239-241): [0]*T + [1] */
+	float egg_batch[3];        /* generateEggs (properties.h:257-266): (uint)(BoxMuller(mean [0], sd [1]) + 0.5f),
+	                              times pow([2], batches) */
+	float lay_probability;     /* a gravid tries to lay with U < p (kernel/updateAgents.cl:233) */
+	float bite_human;          /* BITE_HUMAN_PROBABILITY (properties.h:12, updateAgents.cl:15-17) */
+	float bite_indoor;         /* BITE_INDOOR_PROBABILITY (properties.h:13) */
+	float rest_indoor;         /* REST_INDOOR_PROBABILITY (properties.h:14) */
+	uint32_t window[8];        /* transition-time predicates: bit h set if the transition out of state
+	                              egg, larva, pupa, immature, mate seeking, BMS, BMD may happen at world
+	                              hour h, and [7]: gravidsEggLayTime (properties.h:68-70 ... 280-282) */
+	uint32_t reserved;         /* 0 */
+} sampo_species;
+
+/* include/properties.h as a table. */
+void sampo_species_default(sampo_species *out);
+
+/* Check a table: hatch thresholds ascending, dev_temp_range[0] integral and
+ * <= [1], sds >= 0, probabilities in [0, 1], windows within 24 bits. */
+int sampo_species_check(const sampo_species *sp);
+
+/* Read a species file: properties.h values overridden by lines
+ * "name v1 v2 ..." naming a member of sampo_species (windows as
+ * "window_<egg|larva|pupa|immature|mate_seeking|bms|bmd|lay>" followed by
+ * hours and ranges, e.g. "0-6 18-23", or "all" / "none"); '#' starts a
+ * comment.  The result is checked with sampo_species_check(). */
+int sampo_species_read(const char *path, sampo_species *out);
+
 typedef struct {
 	uint32_t mode;          /* SAMPO_MODE_* */
 	uint32_t seed;          /* srand() seed of the host seed stream (src/abms.c:611) */
@@ -105,6 +150,7 @@ typedef struct {
 	 * agents dense in range order (sum of range sizes). */
 	const sampo_layout *init_layout;
 	const sampo_agent *init_agents;
+	const sampo_species *species; /* NULL: include/properties.h */
 } sampo_scenario;
 
 typedef struct sampo_engine sampo_engine;

@@ -91,10 +91,12 @@ typedef struct _icl_buffer {
 
 typedef enum { ICL_SOURCE, ICL_BINARY, ICL_STRING, ICL_NO_CACHE } icl_create_kernel_flag;
 
-/* include/lib_icl.h:85-88: a registry entry bound to a device */
+/* include/lib_icl.h:85-88: a registry entry bound to a device, with the
+ * device copy of the species table its build options selected */
 typedef struct _icl_kernel {
 	int id;
 	icl_device *dev;
+	void *species;
 } icl_kernel;
 
 /* include/lib_icl.h:90-93: start/stop HIP events of the last command */
@@ -143,8 +145,12 @@ void icl_release_buffer(icl_buffer *buf);
 void icl_release_buffers(cl_uint num, ...);
 
 /* ---- kernels: include/lib_icl.h:121-123 ----
- * file_name and build_options are accepted for compatibility of the call;
- * kernel_name selects the registered gfx950 kernel.  icl_run_kernel takes
+ * kernel_name selects the registered gfx950 kernel and file_name, if given,
+ * must name the reference file that holds it.  Of build_options only
+ * -DSPECIES=<x> is read (src/abms.c:565-568): a name ending in properties.h
+ * selects include/properties.h, anything else names a species file
+ * (sampo_species_read, sampo_engine.h), looked up as given and then under
+ * each -I directory of the options; without it, properties.h.  icl_run_kernel takes
  * num_args pairs (size_t size, void *value): size 0 passes an icl_buffer*,
  * (n, NULL) requests n bytes of local memory, anything else is a by-value
  * argument of that size (src/lib_icl.c:668-704). */

@@ -25,10 +25,20 @@ ENV = np.dtype([("bloodmealSuccess", "<f4"), ("ITNValue", "<f4"), ("IRSValue", "
                 ("carryingCapacity", "<f4"), ("larvacideValue", "<f4"), ("oviTrapValue", "<f4")])
 
 
+# or_species (same layout as the product's species table)
+SPECIES = np.dtype([
+    ("egg_incubation", "<f4", (2,)), ("hatch_threshold", "<f4", (4,)), ("hatch_slope", "<f4", (5,)),
+    ("hatch_offset", "<f4", (5,)), ("larva_delay", "<f4", (2,)), ("dev_temp_range", "<f4", (2,)),
+    ("dev_rates", "<f4", (32,)), ("pupa_delay", "<f8", (2,)), ("immature_delay", "<f4", (2,)),
+    ("mate_seeking_delay", "<f4"), ("bms_delay", "<f4"), ("bmd_delay", "<f4", (2,)), ("egg_batch", "<f4", (3,)),
+    ("lay_probability", "<f4"), ("bite_human", "<f4"), ("bite_indoor", "<f4"), ("rest_indoor", "<f4"),
+    ("window", "<u4", (8,)), ("reserved", "<u4")])
+
+
 class Config(C.Structure):
     _fields_ = [("mode", C.c_int), ("scan_path", C.c_int), ("threads", C.c_int),
                 ("capacity", C.c_uint32), ("steps", C.c_uint32), ("first_step", C.c_uint32),
-                ("dmr", C.c_float * 4), ("temp", C.c_void_p), ("env", C.c_void_p)]
+                ("dmr", C.c_float * 4), ("temp", C.c_void_p), ("env", C.c_void_p), ("species", C.c_void_p)]
 
 
 class Pop(C.Structure):
@@ -79,6 +89,14 @@ def lib():
         L.or_hybrid_taus.restype = C.c_float
         L.or_hmr_reference.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_float]
         L.or_hmr_reference.restype = C.c_float
+        L.or_species_default.argtypes = [vp]
+        L.or_species_default.restype = None
+        L.or_sp_eggs_hatching.argtypes = [vp, C.c_float]
+        L.or_sp_eggs_hatching.restype = C.c_float
+        L.or_sp_larva_development.argtypes = [vp, C.c_float]
+        L.or_sp_larva_development.restype = C.c_float
+        L.or_sp_egg_delay.argtypes = [vp, C.c_float, C.c_float]
+        L.or_sp_egg_delay.restype = C.c_float
         _lib = L
     return _lib
 
@@ -94,6 +112,18 @@ def seed_stream(seed, n):
     return out
 
 
+def default_species():
+    """include/properties.h as the oracle restates it (SPECIES record)."""
+    sp = np.zeros(1, SPECIES)
+    lib().or_species_default(sp.ctypes.data)
+    return sp[0]
+
+
+def as_species(sp):
+    """A one-element SPECIES array holding the bytes of any same-layout record."""
+    return np.frombuffer(np.asarray(sp).tobytes(), dtype=SPECIES).copy()
+
+
 def bolt_scan(keys, vals, wx=128):
     k = np.ascontiguousarray(keys, dtype=np.uint32)
     v = np.array(vals, dtype=np.int32, copy=True)
@@ -105,7 +135,7 @@ class Sim:
     """One oracle run (reference restatement, array-of-structs layout)."""
 
     def __init__(self, *, temp, env, capacity, seed=1, n_eggs=None, layout=None, agents=None,
-                 mode=0, dmr=(0, 0, 0, 0), scan_path=0, threads=1):
+                 mode=0, dmr=(0, 0, 0, 0), scan_path=0, threads=1, species=None):
         self.temp = np.ascontiguousarray(temp, dtype=np.float32)
         env = np.ascontiguousarray(env, dtype=ENV)
         if env.shape[0] == 1:
@@ -117,6 +147,9 @@ class Sim:
         cfg.dmr = (C.c_float * 4)(*dmr)
         cfg.temp = self.temp.ctypes.data
         cfg.env = self.env.ctypes.data
+        if species is not None:
+            self.species = as_species(species)
+            cfg.species = self.species.ctypes.data
         self.cfg = cfg
         self.h = lib().or_sim_create()
         if layout is None:

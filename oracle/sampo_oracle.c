@@ -55,50 +55,83 @@ void or_seed_stream(uint32_t seed, uint32_t n, uint32_t *out)
 }
 
 /* ------------------------------------------------ species plug-in restated
- * include/properties.h */
+ * include/properties.h, with each constant of the header as a table entry */
+
+#define NIGHT_HOURS 0xfc007fu /* worldTime <= 6u || worldTime >= 18u */
+#define ALL_HOURS 0xffffffu   /* return true; */
+
+static const or_species k_properties_h = {
+	{-0.923f, 60.923f},                                            /* properties.h:29-31 */
+	{0.5f, 0.85f, 0.90f, 0.94f},                                   /* properties.h:39-60 */
+	{48.0f, 68.5714f, 480.0f, 600.00f, 2400.0f},
+	{0.0f, -10.2857f, -360.0f, -468.0f, -2160.0f},
+	{1.0f, 0.1f},                                                  /* properties.h:87-91 */
+	{12.0f, 48.0f},                                                /* properties.h:105-109 */
+	{0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f,
+	 0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f,
+	 0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f,
+	 0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f,
+	 0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f,
+	 0.00000533086f, 0.0000000840986f},                             /* properties.h:94-98 */
+	{-0.923, 60.923},                                              /* properties.h:133-135, unsuffixed */
+	{-2.667f, 120.0f},                                             /* properties.h:159-161 */
+	1.0f,                                                          /* properties.h:185-187 */
+	0.0f,                                                          /* properties.h:212-214 */
+	{-1.231f, 77.231f},                                            /* properties.h:239-241 */
+	{170.0f, 30.0f, 0.8f},                                         /* properties.h:257-266 */
+	0.25f,                                                         /* updateAgents.cl:233 */
+	1.0f, 1.0f, 1.0f,                                              /* properties.h:12-14 */
+	/* eggs :68-70, larvae :117-119, pupae :143-145, immatures :169-171,
+	 * mate seeking :195-197 (== 18), BMS :222-224, BMD :249-251, lay :280-282 */
+	{ALL_HOURS, NIGHT_HOURS, NIGHT_HOURS, ALL_HOURS, 1u << 18, NIGHT_HOURS, ALL_HOURS, NIGHT_HOURS},
+	0u,
+};
+
+void or_species_default(or_species *out) { *out = k_properties_h; }
 
 /* properties.h:29-31 */
-static float eggs_incubation(float t) { return -0.923f * t + 60.923f; }
+static float eggs_incubation(const or_species *sp, float t) { return sp->egg_incubation[0] * t + sp->egg_incubation[1]; }
 
 /* properties.h:39-60: piecewise-linear hatching time, rounded to an integer */
-float or_eggs_hatching(float p)
+float or_sp_eggs_hatching(const or_species *sp, float p)
 {
 	float m, b;
-	if (p < 0.5f) { m = 48.0f; b = 0.0f; }
-	else if (p < 0.85f) { m = 68.5714f; b = -10.2857f; }
-	else if (p < 0.90f) { m = 480.0f; b = -360.0f; }
-	else if (p < 0.94f) { m = 600.00f; b = -468.0f; }
-	else { m = 2400.0f; b = -2160.0f; }
+	if (p < sp->hatch_threshold[0]) { m = sp->hatch_slope[0]; b = sp->hatch_offset[0]; }
+	else if (p < sp->hatch_threshold[1]) { m = sp->hatch_slope[1]; b = sp->hatch_offset[1]; }
+	else if (p < sp->hatch_threshold[2]) { m = sp->hatch_slope[2]; b = sp->hatch_offset[2]; }
+	else if (p < sp->hatch_threshold[3]) { m = sp->hatch_slope[3]; b = sp->hatch_offset[3]; }
+	else { m = sp->hatch_slope[4]; b = sp->hatch_offset[4]; }
 	return (float)sc_f2u(m * p + b + 0.5f);
 }
 
-/* properties.h:94-98: larval development per hour, 12..43 degC */
-static const float dev_rates[32] = {
-	0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f,
-	0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f,
-	0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f,
-	0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f,
-	0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f,
-	0.00000533086f, 0.0000000840986f };
+/* initAgents.cl:57 */
+float or_sp_egg_delay(const or_species *sp, float t, float p)
+{
+	return eggs_incubation(sp, t) + or_sp_eggs_hatching(sp, p);
+}
 
 /* properties.h:105-109; the index is clamped to the table (the reference
  * reads past its end for T >= 44 degC). */
-float or_larva_development(float t)
+float or_sp_larva_development(const or_species *sp, float t)
 {
-	float c = sc_minf(sc_maxf(t, 12.0f), 48.0f);
-	int i = (int)c - 12;
+	float c = sc_minf(sc_maxf(t, sp->dev_temp_range[0]), sp->dev_temp_range[1]);
+	int i = (int)c - (int)sp->dev_temp_range[0];
 	if (i > 31) i = 31;
-	return dev_rates[i];
+	return sp->dev_rates[i];
 }
 
 /* properties.h:133-135: evaluated in binary64 (unsuffixed literals) */
-float or_pupa_delay(float t) { return (float)(-0.923 * (double)t + 60.923); }
+static float pupa_delay(const or_species *sp, float t) { return (float)(sp->pupa_delay[0] * (double)t + sp->pupa_delay[1]); }
 /* properties.h:159-161 */
-static float immature_delay(float t) { return -2.667f * t + 120.0f; }
+static float immature_delay(const or_species *sp, float t) { return sp->immature_delay[0] * t + sp->immature_delay[1]; }
 /* properties.h:239-241 */
-static float bmd_delay(float t) { return -1.231f * t + 77.231f; }
-/* properties.h:117-119 etc.: night window */
-static int night(uint32_t h) { return h <= 6u || h >= 18u; }
+static float bmd_delay(const or_species *sp, float t) { return sp->bmd_delay[0] * t + sp->bmd_delay[1]; }
+/* the transition-time predicates, properties.h:68-70 ... 280-282 */
+static int transition_time(const or_species *sp, int s, uint32_t h) { return (int)((sp->window[s] >> h) & 1u); }
+
+float or_eggs_hatching(float p) { return or_sp_eggs_hatching(&k_properties_h, p); }
+float or_larva_development(float t) { return or_sp_larva_development(&k_properties_h, t); }
+float or_pupa_delay(float t) { return pupa_delay(&k_properties_h, t); }
 
 /* ------------------------------------------------------------ mortality
  * kernel/killAgents.cl:84-147 */
@@ -148,7 +181,8 @@ static void k_init_agents(or_sim *s, int buf, uint32_t n_new, float t)
 	or_agent *ag = s->agents[buf];
 	or_age *ge = s->ages[buf];
 	or_state *st = s->states[buf];
-	float inc = eggs_incubation(t);
+	const or_species *sp = &s->sp;
+	float inc = eggs_incubation(sp, t);
 	#pragma omp parallel for schedule(static) num_threads(s->cfg.threads > 0 ? s->cfg.threads : 1)
 	for (int64_t id = 0; id < (int64_t)n_new; ++id) {
 		uint32_t g = (uint32_t)id;
@@ -163,7 +197,7 @@ static void k_init_agents(or_sim *s, int buf, uint32_t n_new, float t)
 		ag[g].cycleLength = 0u;
 		ag[g].aux = sc_bits_f(0.0f);
 		float prob = or_hybrid_taus(&sd[2], g);
-		ag[g].delay = inc + or_eggs_hatching(prob);
+		ag[g].delay = inc + or_sp_eggs_hatching(sp, prob);
 		ag[g].numEggBatches = 0u;
 	}
 }
@@ -197,11 +231,11 @@ static void k_kill_agents(or_sim *s, uint32_t global, float cc)
 }
 
 /* include/properties.h:257-266 */
-static uint32_t generate_eggs(const or_seeds *sd, uint32_t g, uint32_t *neb)
+static uint32_t generate_eggs(const or_species *sp, const or_seeds *sd, uint32_t g, uint32_t *neb)
 {
 	float u0 = or_hybrid_taus(&sd[0], g), u1 = or_hybrid_taus(&sd[1], g);
-	uint32_t eggs = sc_f2u(sc_box_muller(u0, u1, 170.0f, 30.0f) + 0.5f);
-	eggs = sc_f2u((float)eggs * sc_powf(0.8f, (float)*neb) + 0.5f);
+	uint32_t eggs = sc_f2u(sc_box_muller(u0, u1, sp->egg_batch[0], sp->egg_batch[1]) + 0.5f);
+	eggs = sc_f2u((float)eggs * sc_powf(sp->egg_batch[2], (float)*neb) + 0.5f);
 	++(*neb);
 	return eggs;
 }
@@ -215,6 +249,7 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 	or_age *ge = s->ages[s->cur];
 	or_state *st = s->states[s->cur];
 	const or_env env = *envp;
+	const or_species *sp = &s->sp;
 	const float dt = 1.0f; /* src/abms.c:15 */
 	uint32_t tb = (uint32_t)s->enb.totalBiomass;
 	uint32_t new_eggs = 0, bites = 0, ibites = 0, ncyc = 0, scyc = 0;
@@ -236,10 +271,11 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 			ae.csd = (float)((double)ae.csd + (double)cond * x);
 		}
 		if (g < p->r[R_EGG].end) { /* :72-87 */
-			if (ae.hoursInState >= a.delay) {
+			if (ae.hoursInState >= a.delay && transition_time(sp, 0, h)) {
 				ae.ageInHours = 24.0f;
 				ae.hoursInState = 0.0f;
-				a.delay = sc_box_muller(or_hybrid_taus(&sd[1], g), or_hybrid_taus(&sd[2], g), 1.0f, 0.1f);
+				a.delay = sc_box_muller(or_hybrid_taus(&sd[1], g), or_hybrid_taus(&sd[2], g), sp->larva_delay[0],
+				                        sp->larva_delay[1]);
 				as.state = OR_LARVA;
 				ag[g] = a;
 				st[g] = as;
@@ -255,11 +291,11 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 					continue;
 				}
 			}
-			float cl = sc_float(a.aux) + or_larva_development(t);
+			float cl = sc_float(a.aux) + or_sp_larva_development(sp, t);
 			a.aux = sc_bits_f(cl);
-			if (cl >= a.delay && night(h)) {
+			if (cl >= a.delay && transition_time(sp, 1, h)) {
 				ae.hoursInState = 0.0f;
-				a.delay = or_pupa_delay(t);
+				a.delay = pupa_delay(sp, t);
 				as.state = OR_PUPA;
 				st[g] = as;
 			}
@@ -268,10 +304,10 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 			continue;
 		}
 		if (g < p->r[R_PUPA].end) { /* :115-129 */
-			if (ae.hoursInState >= a.delay && night(h)) {
+			if (ae.hoursInState >= a.delay && transition_time(sp, 2, h)) {
 				ae.ageInHours = 0.0f;
 				ae.hoursInState = 0.0f;
-				a.delay = immature_delay(t);
+				a.delay = immature_delay(sp, t);
 				as.state = OR_IMMATURE;
 				ag[g] = a;
 				st[g] = as;
@@ -280,8 +316,8 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 			continue;
 		}
 		if (g < p->r[R_IMM].end) { /* :131-143 */
-			if (ae.hoursInState >= a.delay) {
-				a.delay = 1.0f;
+			if (ae.hoursInState >= a.delay && transition_time(sp, 3, h)) {
+				a.delay = sp->mate_seeking_delay;
 				ae.hoursInState = 0.0f;
 				as.state = OR_MATESEEKING;
 				ag[g] = a;
@@ -291,9 +327,9 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 			continue;
 		}
 		if (g < p->r[R_MS].end) { /* :145-157 */
-			if (ae.hoursInState >= a.delay && ae.isFemale && h == 18u) {
+			if (ae.hoursInState >= a.delay && ae.isFemale && transition_time(sp, 4, h)) {
 				ae.hoursInState = 0.0f;
-				a.delay = 0.0f;
+				a.delay = sp->bms_delay;
 				as.state = OR_BMS;
 				st[g] = as;
 				ag[g] = a;
@@ -303,20 +339,21 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 		}
 		if (g < p->r[R_BMS].end) { /* :159-197 */
 			++a.cycleLength;
-			if (ae.hoursInState >= a.delay && night(h)) {
+			if (ae.hoursInState >= a.delay && transition_time(sp, 5, h)) {
 				if (or_hybrid_taus(&sd[1], g) <= env.bloodmealSuccess) {
-					float itn = env.ITNValue * 1.0f * 1.0f;
+					float itn = env.ITNValue * sp->bite_human * sp->bite_indoor;
 					if (itn > 0.0f && or_hybrid_taus(&sd[2], g) <= itn) {
 						as.dead = 1;
 						st[g] = as;
 						continue;
 					}
-					a.humanBloodmealCount += 1u;
+					int human = sp->bite_human < 1.0f ? (or_hybrid_taus(&sd[3], g) <= sp->bite_human) : 1;
+					a.humanBloodmealCount += 1u * (uint32_t)human;
 					bites += 1u;
 					ae.hoursInState = 0.0f;
-					a.delay = bmd_delay(t);
+					a.delay = bmd_delay(sp, t);
 					as.state = OR_BMD;
-					if (ae.csd >= 1.0f) ibites += 1u;
+					if (ae.csd >= 1.0f && human) ibites += 1u;
 					st[g] = as;
 				}
 			}
@@ -325,16 +362,16 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 			continue;
 		}
 		if (g < p->r[R_BMD].end) { /* :199-228 */
-			float irs = env.IRSValue * 1.0f;
+			float irs = env.IRSValue * sp->rest_indoor;
 			if (irs > 0.0f && or_hybrid_taus(&sd[2], g) <= irs) {
 				as.dead = 1;
 				st[g] = as;
 				continue;
 			}
 			++a.cycleLength;
-			if (ae.hoursInState >= a.delay) {
+			if (ae.hoursInState >= a.delay && transition_time(sp, 6, h)) {
 				if (a.availableEggs == 0u && ae.isFemale)
-					a.availableEggs = generate_eggs(sd + 1, g, &a.numEggBatches);
+					a.availableEggs = generate_eggs(sp, sd + 1, g, &a.numEggBatches);
 				ae.hoursInState = 0.0f;
 				a.aux = 0u; /* ovipositionAttemps = 0 overwrites the union */
 				as.state = OR_GRAVID;
@@ -346,8 +383,8 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 		}
 		/* gravid :230-269 */
 		++a.cycleLength;
-		if (night(h)) {
-			if (or_hybrid_taus(&sd[1], g) < 0.25f) {
+		if (transition_time(sp, 7, h)) {
+			if (or_hybrid_taus(&sd[1], g) < sp->lay_probability) {
 				if (env.oviTrapValue > 0.0f && or_hybrid_taus(&sd[2], g) <= env.oviTrapValue) {
 					as.dead = 1;
 					a.availableEggs = 0;
@@ -365,7 +402,7 @@ static void k_update_agents(or_sim *s, uint32_t global, const or_env *envp, floa
 					scyc += a.cycleLength;
 					a.cycleLength = 0;
 					ae.hoursInState = 0.0f;
-					a.delay = 0.0f;
+					a.delay = sp->bms_delay;
 					as.state = OR_BMS;
 					st[g] = as;
 				}
@@ -587,6 +624,8 @@ static int alloc_sim(or_sim *s, const or_config *cfg)
 {
 	memset(s, 0, sizeof(*s));
 	s->cfg = *cfg;
+	s->sp = cfg->species ? *cfg->species : k_properties_h;
+	s->cfg.species = &s->sp;
 	size_t n = (size_t)STORAGE * cfg->capacity + MARGIN;
 	for (int b = 0; b < 2; ++b) {
 		s->agents[b] = (or_agent *)calloc(n, sizeof(or_agent));

@@ -83,6 +83,31 @@ typedef struct {
 	uint32_t numCycles, sumCycles, bites, infectBites; /* previous iteration's bnc */
 } or_stats;
 
+/* A species plug-in (the header of -DSPECIES, src/abms.c:565-568) as a
+ * table: each function of include/properties.h is one member of the
+ * parametric family below, or_species_default() is properties.h itself.
+ * Same layout as the product's sampo_species (312 bytes). */
+typedef struct {
+	float egg_incubation[2];   /* eggsIncubation: [0]*T + [1] */
+	float hatch_threshold[4];  /* eggsHatching: bands p < threshold[i] */
+	float hatch_slope[5], hatch_offset[5];
+	float larva_delay[2];      /* larvaDelay: BoxMuller(mean, sd) */
+	float dev_temp_range[2];   /* larvaDevelopment: clamp range, table starts at [0] */
+	float dev_rates[32];
+	double pupa_delay[2];      /* pupaDelay, binary64 */
+	float immature_delay[2];
+	float mate_seeking_delay;
+	float bms_delay;
+	float bmd_delay[2];
+	float egg_batch[3];        /* generateEggs: mean, sd, decay per batch */
+	float lay_probability;     /* updateAgents.cl:233 */
+	float bite_human, bite_indoor, rest_indoor; /* BITE_HUMAN/BITE_INDOOR/REST_INDOOR_PROBABILITY */
+	uint32_t window[8];        /* transition-time predicates as hour masks; [7] gravidsEggLayTime */
+	uint32_t reserved;
+} or_species;
+
+void or_species_default(or_species *out);
+
 typedef struct {
 	int mode;              /* 0 = reference kill formulas, 1 = per-stage constant mortality */
 	int scan_path;         /* 0 = GPU path (preScan + Bolt), 1 = CPU-device path (serialScan) */
@@ -93,6 +118,7 @@ typedef struct {
 	float dmr[4];          /* mode 1: daily mortality egg, larva, pupa, adult */
 	const float *temp;     /* per-iteration temperature, indexed by step */
 	const or_env *env;     /* per-iteration environment, indexed by step */
+	const or_species *species; /* NULL: include/properties.h */
 } or_config;
 
 typedef struct {
@@ -114,6 +140,7 @@ typedef struct {
 	uint32_t step;       /* iterations done */
 	int status;          /* 0 ok, 1 capacity exceeded, 2 extinct */
 	void *rng;           /* sc_glibc_rand */
+	or_species sp;       /* the species of the run */
 } or_sim;
 
 /* Set up a run.  Reference initial population: n_eggs eggs created by
@@ -160,9 +187,14 @@ float or_sqrtf(float x);
 float or_powf(float x, float y);
 uint32_t or_f2u(float x);
 float or_box_muller(float u0, float u1, float mean, float sd);
+/* properties.h functions of the default species */
 float or_eggs_hatching(float p);
 float or_larva_development(float t);
 float or_pupa_delay(float t);
+/* the same of a given species table */
+float or_sp_eggs_hatching(const or_species *sp, float p);
+float or_sp_larva_development(const or_species *sp, float t);
+float or_sp_egg_delay(const or_species *sp, float t, float p);
 /* hourly mortality rate used by killAgents for range class cls
  * (0 egg, 1 larva, 2 pupa, 3 adult), reference mode */
 float or_hmr_reference(int cls, uint32_t age_days, uint32_t l1de, float cc);

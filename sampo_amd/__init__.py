@@ -54,7 +54,18 @@ class Scenario(C.Structure):
                 ("n_eggs", C.c_uint32), ("max_agents", C.c_uint64), ("phys_slots", C.c_uint64),
                 ("dmr", C.c_float * 4), ("temp", C.POINTER(C.c_float)), ("env", C.POINTER(Env)),
                 ("env_count", C.c_uint32), ("init_layout", C.POINTER(Layout)),
-                ("init_agents", C.POINTER(Agent))]
+                ("init_agents", C.POINTER(Agent)), ("species", C.c_void_p)]
+
+
+# sampo_species: the species plug-in (the -DSPECIES header, include/properties.h) as data
+SPECIES_DTYPE = np.dtype([
+    ("egg_incubation", "<f4", (2,)), ("hatch_threshold", "<f4", (4,)), ("hatch_slope", "<f4", (5,)),
+    ("hatch_offset", "<f4", (5,)), ("larva_delay", "<f4", (2,)), ("dev_temp_range", "<f4", (2,)),
+    ("dev_rates", "<f4", (32,)), ("pupa_delay", "<f8", (2,)), ("immature_delay", "<f4", (2,)),
+    ("mate_seeking_delay", "<f4"), ("bms_delay", "<f4"), ("bmd_delay", "<f4", (2,)), ("egg_batch", "<f4", (3,)),
+    ("lay_probability", "<f4"), ("bite_human", "<f4"), ("bite_indoor", "<f4"), ("rest_indoor", "<f4"),
+    ("window", "<u4", (8,)), ("reserved", "<u4")])
+WINDOWS = ("egg", "larva", "pupa", "immature", "mate_seeking", "bms", "bmd", "lay")
 
 
 AGENT_DTYPE = np.dtype([("ageInHours", "<f4"), ("hoursInState", "<f4"), ("delay", "<f4"),
@@ -107,6 +118,10 @@ def lib():
         L.sampo_seed_stream.restype = None
         L.sampo_gen_temperature.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float, vp]
         L.sampo_gen_population.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.POINTER(Layout), vp]
+        L.sampo_species_default.argtypes = [vp]
+        L.sampo_species_default.restype = None
+        L.sampo_species_check.argtypes = [vp]
+        L.sampo_species_read.argtypes = [C.c_char_p, vp]
         _lib = L
     return _lib
 
@@ -156,9 +171,50 @@ def make_env(bloodmeal=0.25, itn=0.0, irs=0.0, cc=160000.0, larvicide=0.0, ovitr
                     dtype=[(n, "<f4") for n, _ in Env._fields_])
 
 
+def default_species():
+    """include/properties.h as a SPECIES_DTYPE record."""
+    sp = np.zeros(1, SPECIES_DTYPE)
+    lib().sampo_species_default(sp.ctypes.data)
+    return sp[0]
+
+
+def check_species(sp):
+    """True if the library accepts the table (sampo_species_check)."""
+    a = np.array([sp], dtype=SPECIES_DTYPE)
+    return lib().sampo_species_check(a.ctypes.data) == 0
+
+
+def read_species(path):
+    """A species file (sampo_species_read): properties.h with the lines of the file applied."""
+    sp = np.zeros(1, SPECIES_DTYPE)
+    _check(lib().sampo_species_read(str(path).encode(), sp.ctypes.data))
+    return sp[0]
+
+
+def _hours(mask):
+    hs = [h for h in range(24) if (int(mask) >> h) & 1]
+    return " ".join(str(h) for h in hs) if hs else "none"
+
+
+def write_species(path, sp):
+    """Write every member of a species record in the species file format."""
+    lines = ["# species table (sampo_species_read format)"]
+    for name in SPECIES_DTYPE.names:
+        if name == "reserved":
+            continue
+        if name == "window":
+            lines += [f"window_{w} {_hours(sp['window'][i])}" for i, w in enumerate(WINDOWS)]
+            continue
+        v = np.atleast_1d(sp[name])  # shortest text that reads back to the same binary32 / binary64
+        lines.append(name + " " + " ".join(str(x) for x in v))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def scenario(*, mode=MODE_REFERENCE, seed=1, horizon, temp, env, n_eggs=0, max_agents, phys_slots=0,
-             dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):
-    """A sampo_scenario and the arrays it points into (keep them alive)."""
+             dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None, species=None):
+    """A sampo_scenario and the arrays it points into (keep them alive).
+    species: a SPECIES_DTYPE record, or None for include/properties.h."""
     temp = np.ascontiguousarray(temp, dtype=np.float32)
     env = np.ascontiguousarray(env)
     if temp.shape[0] < horizon:
@@ -179,6 +235,10 @@ def scenario(*, mode=MODE_REFERENCE, seed=1, horizon, temp, env, n_eggs=0, max_a
         sc.init_layout = C.pointer(lay)
         sc.init_agents = C.cast(agents.ctypes.data, C.POINTER(Agent))
         keep += [lay, agents]
+    if species is not None:
+        sp = np.array([species], dtype=SPECIES_DTYPE)
+        sc.species = sp.ctypes.data
+        keep.append(sp)
     return sc, keep
 
 
@@ -314,7 +374,7 @@ class AbmsParams(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("n_eggs", C.c_uint32), ("max_agents", C.c_uint32), ("steps", C.c_uint32),
                 ("serial_scan", C.c_uint32), ("temp", C.POINTER(C.c_float)), ("env", C.POINTER(Env)),
                 ("env_count", C.c_uint32), ("mode", C.c_uint32), ("dmr", C.c_float * 4),
-                ("init_layout", C.POINTER(Layout)), ("init_agents", C.POINTER(Agent))]
+                ("init_layout", C.POINTER(Layout)), ("init_agents", C.POINTER(Agent)), ("species", C.c_char_p)]
 
 
 class AbmsResult(C.Structure):
@@ -350,8 +410,9 @@ def icl_lib():
 
 
 def run_abms(*, seed, n_eggs=0, max_agents, steps, temp, env, serial_scan=False, export=True, mode=MODE_REFERENCE,
-             dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None):
+             dmr=(0.0, 0.0, 0.0, 0.0), layout=None, agents=None, species=None):
     """Run the reference driver loop on the GPU through the icl boundary.
+    species: the speciesHeader argument (None: properties.h, else a species file).
 
     Returns a dict with rows, status, steps_done, agent_steps, layout,
     histogram, loop_ms and (export=True) the final AoS agent arrays."""
@@ -361,6 +422,8 @@ def run_abms(*, seed, n_eggs=0, max_agents, steps, temp, env, serial_scan=False,
                    temp.ctypes.data_as(C.POINTER(C.c_float)), C.cast(env.ctypes.data, C.POINTER(Env)), env.shape[0])
     p.mode = mode
     p.dmr = (C.c_float * 4)(*dmr)
+    if species is not None:
+        p.species = str(species).encode()
     keep = None
     if layout is not None:
         lay = Layout()

@@ -27,8 +27,8 @@ HIP_FLAGS = [
 ]
 
 TARGETS = {
-    "libsampo_engine.so": ["sampo_engine.hip"],
-    "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c", "sampo_report.c"],
+    "libsampo_engine.so": ["sampo_engine.hip", "sampo_species.c"],
+    "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c", "sampo_report.c", "sampo_species.c"],
 }
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 

@@ -41,6 +41,7 @@ typedef struct {
 	icl_kernel *calcL1dePerGroup, *calcL1deTotal, *calcFemalesPerGroup, *calcFemalesTotal;
 	icl_kernel *killAgents, *updateAgents, *oldToNewAgents, *preScan, *serialScan, *createEggs;
 	float hmr[4]; /* configuration mode: hourly mortality egg, larva, pupa, adult */
+	char opts[1100]; /* kernelBuildArgs (src/abms.c:565-568) */
 } Driver;
 
 static size_t roundup(size_t v, size_t m) { return ((v + m - 1) / m) * m; }
@@ -105,7 +106,7 @@ static void upload_population(Driver *d, icl_buffer *agents, icl_buffer *ages, i
 static void create_initial_population(Driver *d, icl_buffer *agents, icl_buffer *ages, icl_buffer *states,
                                       Population *pop, icl_buffer *enbD, icl_buffer *seeds)
 {
-	d->createEggs = icl_create_kernel(d->dev, "kernel/initAgents.cl", "initAgents", "", ICL_SOURCE);
+	d->createEggs = icl_create_kernel(d->dev, "kernel/initAgents.cl", "initAgents", d->opts, ICL_SOURCE);
 	if (d->p->mode == SAMPO_MODE_CONFIG) {
 		upload_population(d, agents, ages, states, pop);
 		return;
@@ -259,6 +260,14 @@ int sampo_abms_run(const sampo_abms_params *p, sampo_abms_result *res)
 	Driver D;
 	memset(&D, 0, sizeof(D));
 	D.p = p;
+	{ /* readArguments, src/abms.c:557-583: the species header goes into every kernel's build options */
+		const char *sp = p->species && *p->species ? p->species : "properties.h";
+		size_t n = strlen(sp), k = strlen("properties.h");
+		sampo_species check;
+		if (strlen(sp) > 1024) return SAMPO_E_ARG;
+		if ((n < k || strcmp(sp + n - k, "properties.h") != 0) && sampo_species_read(sp, &check)) return SAMPO_E_ARG;
+		snprintf(D.opts, sizeof D.opts, "-Iinclude -DSPECIES=%s", sp);
+	}
 	D.maxAgents = p->max_agents;
 	for (int c = 0; c < 4; ++c) D.hmr[c] = sm_hmr(p->dmr[c]);
 	sc_glibc_seed(&D.rng, p->seed);
@@ -302,17 +311,18 @@ int sampo_abms_run(const sampo_abms_params *p, sampo_abms_result *res)
 	icl_buffer *p2 = icl_create_buffer(dev, CL_MEM_READ_WRITE, (cap + 2) * sizeof(cl_int));
 	icl_buffer *p3 = icl_create_buffer(dev, CL_MEM_READ_WRITE, (cap + 2) * sizeof(cl_int));
 	icl_write_buffer_offset(popD, CL_TRUE, sizeof(Population), sizeof(Population), &popH, NULL, NULL);
-	D.calcL1dePerGroup = icl_create_kernel(dev, "kernel/calcL1de.cl", "calcL1dePerGroup", "", ICL_SOURCE);
-	D.calcL1deTotal = icl_create_kernel(dev, "kernel/calcL1de.cl", "calcL1deTotal", "", ICL_SOURCE);
-	D.calcFemalesPerGroup = icl_create_kernel(dev, "kernel/calcGender.cl", "calcFemalesPerGroup", "", ICL_SOURCE);
-	D.calcFemalesTotal = icl_create_kernel(dev, "kernel/calcGender.cl", "calcFemalesTotal", "", ICL_SOURCE);
+	const char *o = D.opts;
+	D.calcL1dePerGroup = icl_create_kernel(dev, "kernel/calcL1de.cl", "calcL1dePerGroup", o, ICL_SOURCE);
+	D.calcL1deTotal = icl_create_kernel(dev, "kernel/calcL1de.cl", "calcL1deTotal", o, ICL_SOURCE);
+	D.calcFemalesPerGroup = icl_create_kernel(dev, "kernel/calcGender.cl", "calcFemalesPerGroup", o, ICL_SOURCE);
+	D.calcFemalesTotal = icl_create_kernel(dev, "kernel/calcGender.cl", "calcFemalesTotal", o, ICL_SOURCE);
 	D.killAgents = icl_create_kernel(dev, "kernel/killAgents.cl",
-	                                 p->mode == SAMPO_MODE_CONFIG ? "killAgentsStaged" : "killAgents", "", ICL_SOURCE);
-	D.updateAgents = icl_create_kernel(dev, "kernel/updateAgents.cl", "updateAgents", "", ICL_SOURCE);
-	D.oldToNewAgents = icl_create_kernel(dev, "kernel/oldToNewAgents.cl", "oldToNewAgents", "", ICL_SOURCE);
-	D.serialScan = icl_create_kernel(dev, "kernel/serialScan.cl", "serialScan", "", ICL_SOURCE);
-	D.preScan = icl_create_kernel(dev, "kernel/preScan.cl", "preScan", "", ICL_SOURCE);
-	segmented_scan_init(2 * LOCAL_SIZE, D.maxAgents, dev, "", ICL_SOURCE);
+	                                 p->mode == SAMPO_MODE_CONFIG ? "killAgentsStaged" : "killAgents", o, ICL_SOURCE);
+	D.updateAgents = icl_create_kernel(dev, "kernel/updateAgents.cl", "updateAgents", o, ICL_SOURCE);
+	D.oldToNewAgents = icl_create_kernel(dev, "kernel/oldToNewAgents.cl", "oldToNewAgents", o, ICL_SOURCE);
+	D.serialScan = icl_create_kernel(dev, "kernel/serialScan.cl", "serialScan", o, ICL_SOURCE);
+	D.preScan = icl_create_kernel(dev, "kernel/preScan.cl", "preScan", o, ICL_SOURCE);
+	segmented_scan_init(2 * LOCAL_SIZE, D.maxAgents, dev, o, ICL_SOURCE);
 
 	icl_timer *loop = icl_init_timer(ICL_MILLI);
 	icl_start_timer(loop);
@@ -359,7 +369,7 @@ int sampo_abms_run(const sampo_abms_params *p, sampo_abms_result *res)
 	segmented_scan_release();
 
 	/* storeHistogram, src/abms.c:160-183 */
-	icl_kernel *hk = icl_create_kernel(dev, "kernel/ageHistogram.cl", "ageHistogram", "", ICL_SOURCE);
+	icl_kernel *hk = icl_create_kernel(dev, "kernel/ageHistogram.cl", "ageHistogram", D.opts, ICL_SOURCE);
 	cl_uint zero[100];
 	memset(zero, 0, sizeof(zero));
 	icl_write_buffer(buff, CL_TRUE, 100 * sizeof(cl_uint), zero, NULL, NULL);

@@ -92,14 +92,33 @@ struct IterState {
 	uint32_t nseg, nchunk, enew, nb_set;
 	uint32_t cs;           /* agents per chunk of this iteration */
 	uint32_t seg0_done;    /* chunk seg0 already filled by the fold */
-	float nb_temp;
 	uint32_t tb, h, seed_set;
 	float temp;
 	sampo_env env;
 	float hmr_egg, hmr_pupa;
-	uint32_t pad;
+	/* the species functions (include/properties.h) at this iteration's
+	 * temperature, environment and hour, from the instance's SpRow table */
+	uint32_t open;         /* bit s: state s may leave now; bit 7: gravids may lay */
+	float dev_rate;        /* larvaDevelopment(T) */
+	float pupa_dly, imm_dly, bmd_dly;
+	float nb_inc;          /* eggsIncubation at the newborns' laying temperature */
+	float itn_kill, irs_kill; /* ITN x BITE_HUMAN x BITE_INDOOR, IRS x REST_INDOOR (updateAgents.cl:166,200) */
+	/* species constants, written into both parities at admission; larva_* and
+	 * batch_* are read as arrays through LDS pointers and must stay adjacent */
+	float ms_dly, bms_dly, larva_mean, larva_sd, batch_mean, batch_sd, batch_decay, lay_p, bite_human;
+	float hatch_thr[4], hatch_m[5], hatch_b[5];
 	double csd_inc;        /* 1/((111/(T-16))*14) of updateAgents.cl:69-70, in binary64 */
 	float hmr_larva[100];
+};
+
+/* Species values of one iteration, evaluated on the host at admission from
+ * its temperature, environment and hour of day: the k_prep work per
+ * iteration stays one 32-byte load. */
+struct SpRow {
+	float dev_rate, pupa_dly, imm_dly, bmd_dly;
+	float inc;             /* eggsIncubation(T), for the eggs laid in this iteration */
+	float itn_kill, irs_kill;
+	uint32_t open;
 };
 
 struct Inst {
@@ -111,6 +130,7 @@ struct Inst {
 	const float *temp;
 	const sampo_env *env;
 	sampo_row *rows;       /* horizon + 1 */
+	const SpRow *sprow;    /* horizon */
 	uint64_t max_agents;
 	uint32_t P, horizon, mode, env_count;
 	uint32_t P_max;        /* physical slots the buffers may grow to */
@@ -130,8 +150,6 @@ struct Ctx {
 };
 
 /* ------------------------------------------------------------ the agent */
-
-__constant__ float c_dev_rates[32] = SM_DEV_RATES_INIT;
 
 struct Ag {
 	float age, his, dly, csd;
@@ -160,19 +178,23 @@ __device__ static inline float rnd(const uint32_t *z, uint32_t g)
 	return sc_hybrid_taus(z[0], z[1], z[2], z[3], g);
 }
 
-__device__ static inline int night(uint32_t h) { return h <= 6u || h >= 18u; }
-
 /* The transcendental paths run only on state transitions; keeping them out
- * of line keeps their binary64 temporaries out of the pass's register budget. */
-__device__ __attribute__((noinline)) float bm_delay(float u0, float u1, float mean, float sd)
+ * of line keeps their binary64 temporaries out of the pass's register budget.
+ * They read their species parameters from the LDS copy of the iteration
+ * state through one 32-bit LDS address: passing the values themselves costs
+ * the caller registers across the call (measured: 2% on the pass). */
+typedef const __attribute__((address_space(3))) float *lds_fp;
+
+/* larvaDelay: p = {mean, sd} */
+__device__ __attribute__((noinline)) float bm_delay(float u0, float u1, lds_fp p)
 {
-	return sc_box_muller(u0, u1, mean, sd);
+	return sc_box_muller(u0, u1, p[0], p[1]);
 }
 
-__device__ __attribute__((noinline)) uint32_t gen_eggs(float u0, float u1, uint32_t nb)
+/* generateEggs: p = {mean, sd, decay} */
+__device__ __attribute__((noinline)) uint32_t gen_eggs(float u0, float u1, uint32_t nb, lds_fp p)
 {
-	uint32_t eggs = sc_f2u(sc_box_muller(u0, u1, 170.0f, 30.0f) + 0.5f);
-	return sc_f2u((float)eggs * sc_powf(0.8f, (float)nb) + 0.5f);
+	return sm_egg_batch(u0, u1, p[0], p[1], p[2], nb);
 }
 
 /* killAgents + updateAgents for one agent in state s at virtual index g.
@@ -195,72 +217,72 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 		int cond = (t > 16.0f && hbc > 0u);
 		a.csd = (float)((double)a.csd + (double)cond * ip.csd_inc);
 	}
-	const uint32_t h = ip.h;
+	const uint32_t open = ip.open;
 	switch (s) {
 	case S_EGG: /* :72-87 */
-		if (a.his >= a.dly) {
+		if (a.his >= a.dly && (open & (1u << S_EGG))) {
 			a.age = 24.0f;
 			a.his = 0.0f;
-			a.dly = bm_delay(rnd(&T.seed[4], g), rnd(&T.seed[8], g), 1.0f, 0.1f);
+			a.dly = bm_delay(rnd(&T.seed[4], g), rnd(&T.seed[8], g), (lds_fp)&ip.larva_mean);
 			return S_LARVA;
 		}
 		return S_EGG;
 	case S_LARVA: { /* :89-113 */
 		if (ip.env.larvacideValue > 0.0f && a.his == 1.0f && rnd(&T.seed[8], g) <= ip.env.larvacideValue)
 			return S_DEAD;
-		/* larvaDevelopment, properties.h:105-109; index clamped to the table */
-		float c = sc_float(a.aux) + c_dev_rates[sm_dev_index(t)];
+		float c = sc_float(a.aux) + ip.dev_rate;
 		a.aux = sc_bits_f(c);
-		if (c >= a.dly && night(h)) {
+		if (c >= a.dly && (open & (1u << S_LARVA))) {
 			a.his = 0.0f;
-			a.dly = (float)(-0.923 * (double)t + 60.923);
+			a.dly = ip.pupa_dly;
 			return S_PUPA;
 		}
 		return S_LARVA;
 	}
 	case S_PUPA: /* :115-129 */
-		if (a.his >= a.dly && night(h)) {
+		if (a.his >= a.dly && (open & (1u << S_PUPA))) {
 			a.age = 0.0f;
 			a.his = 0.0f;
-			a.dly = -2.667f * t + 120.0f;
+			a.dly = ip.imm_dly;
 			return S_IMM;
 		}
 		return S_PUPA;
 	case S_IMM: /* :131-143 */
-		if (a.his >= a.dly) {
-			a.dly = 1.0f;
+		if (a.his >= a.dly && (open & (1u << S_IMM))) {
+			a.dly = ip.ms_dly;
 			a.his = 0.0f;
 			return S_MS;
 		}
 		return S_IMM;
 	case S_MS: /* :145-157 */
-		if (a.his >= a.dly && (a.misc >> 31) && h == 18u) {
+		if (a.his >= a.dly && (a.misc >> 31) && (open & (1u << S_MS))) {
 			a.his = 0.0f;
-			a.dly = 0.0f;
+			a.dly = ip.bms_dly;
 			return S_BMS;
 		}
 		return S_MS;
 	case S_BMS: /* :159-197 */
 		a.cyc += 1u;
-		if (a.his >= a.dly && night(h) && rnd(&T.seed[4], g) <= ip.env.bloodmealSuccess) {
-			float itn = ip.env.ITNValue * 1.0f * 1.0f;
+		if (a.his >= a.dly && (open & (1u << S_BMS)) && rnd(&T.seed[4], g) <= ip.env.bloodmealSuccess) {
+			const float itn = ip.itn_kill;
 			if (itn > 0.0f && rnd(&T.seed[8], g) <= itn) return S_DEAD;
-			a.misc = (a.misc & 0xffff0000u) | ((hbc + 1u) & 0xffffu);
+			const uint32_t human = ip.bite_human < 1.0f ? (rnd(&T.seed[12], g) <= ip.bite_human) : 1u;
+			a.misc = (a.misc & 0xffff0000u) | ((hbc + human) & 0xffffu);
 			acc.bites += 1u;
 			a.his = 0.0f;
-			a.dly = -1.231f * t + 77.231f;
-			if (a.csd >= 1.0f) acc.ibites += 1u;
+			a.dly = ip.bmd_dly;
+			if (a.csd >= 1.0f && human) acc.ibites += 1u;
 			return S_BMD;
 		}
 		return S_BMS;
 	case S_BMD: { /* :199-228 */
-		float irs = ip.env.IRSValue * 1.0f;
+		const float irs = ip.irs_kill;
 		if (irs > 0.0f && rnd(&T.seed[8], g) <= irs) return S_DEAD;
 		a.cyc += 1u;
-		if (a.his >= a.dly) {
+		if (a.his >= a.dly && (open & (1u << S_BMD))) {
 			if (a.avail == 0u && (a.misc >> 31)) { /* generateEggs, properties.h:257-266 */
 				uint32_t nb = (a.misc >> 16) & 0x7fffu;
-				a.avail = gen_eggs(rnd(&T.seed[4], g), rnd(&T.seed[8], g), nb);
+				a.avail = gen_eggs(rnd(&T.seed[4], g), rnd(&T.seed[8], g), nb, (lds_fp)&ip.batch_mean);
 				a.misc = (a.misc & 0x8000ffffu) | (((nb + 1u) & 0x7fffu) << 16);
 			}
 			a.his = 0.0f;
@@ -271,7 +293,7 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	}
 	default: { /* gravid :230-269 */
 		a.cyc += 1u;
-		if (night(h) && rnd(&T.seed[4], g) < 0.25f) {
+		if ((open & (1u << S_GRAVID)) && rnd(&T.seed[4], g) < ip.lay_p) {
 			int dead = 0;
 			if (ip.env.oviTrapValue > 0.0f && rnd(&T.seed[8], g) <= ip.env.oviTrapValue) {
 				dead = 1;
@@ -290,7 +312,7 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 				acc.scyc += a.cyc;
 				a.cyc = 0u;
 				a.his = 0.0f;
-				a.dly = 0.0f;
+				a.dly = ip.bms_dly;
 				return dead ? S_DEAD : S_BMS;
 			}
 		}
@@ -323,9 +345,11 @@ __device__ static inline void store_agent(const Ag &a, uint32_t *b, size_t P, ui
 	b[7 * P + p] = a.misc;
 }
 
-/* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3) at
- * temperature t */
-__device__ static inline void newborn(Ag &a, const uint32_t *z, float t, uint32_t g)
+/* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3);
+ * inc is eggsIncubation at the laying temperature, thr/m/b the eggsHatching
+ * bands */
+__device__ static inline void newborn(Ag &a, const uint32_t *z, float inc, const float *thr, const float *m,
+                                      const float *b, uint32_t g)
 {
 	a.age = 0.0f;
 	a.his = 0.0f;
@@ -336,7 +360,7 @@ __device__ static inline void newborn(Ag &a, const uint32_t *z, float t, uint32_
 	float u3 = sc_hybrid_taus(z[4], z[5], z[6], z[7], g);
 	float u2 = sc_hybrid_taus(z[0], z[1], z[2], z[3], g);
 	a.misc = ((double)u3 <= 0.5) ? 0x80000000u : 0u;
-	a.dly = sm_egg_delay(t, u2);
+	a.dly = inc + sm_hatching(thr, m, b, u2);
 }
 
 __device__ static inline uint32_t wave_sum(uint32_t v)
@@ -705,12 +729,20 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		const uint32_t t = ip.step < I->horizon ? ip.step : I->horizon - 1u;
 		const uint32_t tp = ip.step == 0u ? 0u : (ip.step - 1u < I->horizon ? ip.step - 1u : I->horizon - 1u);
 		ip.temp = I->temp[t];
+		const SpRow sr = I->sprow[t];
+		ip.nb_inc = I->sprow[tp].inc;
 		{
 			float q = 111.0f / (ip.temp - 16.0f);
 			ip.csd_inc = 1.0 / ((double)q * 14.0);
 		}
 		ip.env = I->env[I->env_count == 1u ? 0u : t];
-		ip.nb_temp = I->temp[tp];
+		ip.open = sr.open;
+		ip.dev_rate = sr.dev_rate;
+		ip.pupa_dly = sr.pupa_dly;
+		ip.imm_dly = sr.imm_dly;
+		ip.bmd_dly = sr.bmd_dly;
+		ip.itn_kill = sr.itn_kill;
+		ip.irs_kill = sr.irs_kill;
 		ip.nb_set = ip.step;
 		ip.seed_set = ip.step + 1u;
 		ip.h = sc_f2u((float)ip.step * 1.0f) % 24u;
@@ -1079,7 +1111,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				me[jx].g = s_sg[o];
 				if (me[jx].s < S_DEAD) {
 					if (me[jx].born) {
-						newborn(a[jx], T.nbseed, T.ip.nb_temp, me[jx].g);
+						newborn(a[jx], T.nbseed, T.ip.nb_inc, T.ip.hatch_thr, T.ip.hatch_m, T.ip.hatch_b, me[jx].g);
 					} else {
 						a[jx].age = sc_float(cur[F_AGE][o]);
 						a[jx].his = sc_float(cur[F_HIS][o]);
@@ -1207,14 +1239,19 @@ __device__ static inline uint32_t seg_find_global(const Seg *seg, uint32_t nseg,
 	return lo;
 }
 
+/* what newborn() needs of an iteration, by value */
+struct Hatch {
+	float inc, thr[4], m[5], b[5];
+};
+
 __global__ void k_export(const uint32_t *buf, uint32_t P, const Seg *seg, uint32_t nseg, uint32_t n,
-                         const uint32_t *nbz, float nb_temp, sampo_agent *out)
+                         const uint32_t *nbz, Hatch hb, sampo_agent *out)
 {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
 	const Seg sg = seg[seg_find_global(seg, nseg, k)];
 	Ag a;
-	if (sg.pbase == PB_NEWBORN) newborn(a, nbz, nb_temp, sg.vbase + (k - sg.k0));
+	if (sg.pbase == PB_NEWBORN) newborn(a, nbz, hb.inc, hb.thr, hb.m, hb.b, sg.vbase + (k - sg.k0));
 	else load_agent(a, buf, P, seg_slot(sg.pbase, sg.state, k - sg.k0));
 	sampo_agent r;
 	r.ageInHours = a.age;
@@ -1476,7 +1513,7 @@ extern "C" int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t
 
 /* layout of one instance's bookkeeping allocation (all but the agent buffers) */
 struct SlotLayout {
-	size_t o_seg0, o_seg1, o_ch0, o_ch1, o_cout, o_seeds, o_temp, o_env, o_rows, total;
+	size_t o_seg0, o_seg1, o_ch0, o_ch1, o_cout, o_seeds, o_temp, o_env, o_rows, o_sprow, total;
 	uint32_t cmax, smax;
 };
 
@@ -1495,6 +1532,7 @@ static SlotLayout slot_layout(uint64_t P_max, uint32_t chunk, uint32_t horizon, 
 	L.o_temp = off; off = align256(off + (size_t)horizon * sizeof(float));
 	L.o_env = off; off = align256(off + (size_t)env_count * sizeof(sampo_env));
 	L.o_rows = off; off = align256(off + (size_t)(horizon + 1u) * sizeof(sampo_row));
+	L.o_sprow = off; off = align256(off + (size_t)horizon * sizeof(SpRow));
 	L.total = off;
 	return L;
 }
@@ -1549,6 +1587,25 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	h.chunk_size = e->chunk;
 	h.target_chunks = e->target_chunks ? e->target_chunks : 2u * (uint32_t)e->grid;
 	h.P_max = (uint32_t)round256(sc->max_agents);
+	sampo_species sp;
+	if (sc->species) sp = *sc->species;
+	else sampo_species_default(&sp);
+	if (sampo_species_check(&sp)) return fail(SAMPO_E_ARG, "invalid species table%s%ld");
+	for (int q = 0; q < 2; ++q) { /* species constants of every iteration */
+		IterState &x = h.st[q];
+		x.ms_dly = sp.mate_seeking_delay;
+		x.bms_dly = sp.bms_delay;
+		x.larva_mean = sp.larva_delay[0];
+		x.larva_sd = sp.larva_delay[1];
+		x.batch_mean = sp.egg_batch[0];
+		x.batch_sd = sp.egg_batch[1];
+		x.batch_decay = sp.egg_batch[2];
+		x.lay_p = sp.lay_probability;
+		x.bite_human = sp.bite_human;
+		memcpy(x.hatch_thr, sp.hatch_threshold, sizeof x.hatch_thr);
+		memcpy(x.hatch_m, sp.hatch_slope, sizeof x.hatch_m);
+		memcpy(x.hatch_b, sp.hatch_offset, sizeof x.hatch_b);
+	}
 	for (int c = 0; c < 4; ++c) h.hmr_stage[c] = sm_hmr(sc->dmr[c]);
 	for (uint32_t d = 0; d < 100; ++d) h.hmr_adult[d] = sc->mode == SAMPO_MODE_CONFIG ? h.hmr_stage[3] : sm_hmr_adult(d);
 	IterState &s0 = h.st[0];
@@ -1618,10 +1675,28 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	h.temp = (const float *)(base + L.o_temp);
 	h.env = (const sampo_env *)(base + L.o_env);
 	h.rows = (sampo_row *)(base + L.o_rows);
+	h.sprow = (const SpRow *)(base + L.o_sprow);
+	std::vector<SpRow> rows(sc->horizon);
+	for (uint32_t t = 0; t < sc->horizon; ++t) {
+		const float T = sc->temp[t];
+		const sampo_env &en = sc->env[sc->env_count == 1u ? 0u : t];
+		const uint32_t hr = sc_f2u((float)t * 1.0f) % 24u; /* worldTime, src/abms.c:514 */
+		SpRow &r = rows[t];
+		r.dev_rate = sm_dev_rate(&sp, T);
+		r.pupa_dly = sm_pupa_delay(&sp, T);
+		r.imm_dly = sm_linear(sp.immature_delay, T);
+		r.bmd_dly = sm_linear(sp.bmd_delay, T);
+		r.inc = sm_linear(sp.egg_incubation, T);
+		r.itn_kill = en.ITNValue * sp.bite_human * sp.bite_indoor;
+		r.irs_kill = en.IRSValue * sp.rest_indoor;
+		r.open = 0u;
+		for (int q = 0; q < 8; ++q) r.open |= (uint32_t)sm_open(&sp, q, hr) << q;
+	}
 	err = hipMemcpy((void *)h.seeds, seeds.data(), seeds_n * sizeof(uint32_t), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy((void *)h.temp, sc->temp, (size_t)sc->horizon * sizeof(float), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy((void *)h.env, sc->env, (size_t)sc->env_count * sizeof(sampo_env), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemset(h.rows, 0, (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
+	if (err == hipSuccess) err = hipMemcpy((void *)h.sprow, rows.data(), rows.size() * sizeof(SpRow), hipMemcpyHostToDevice);
 	if (err == hipSuccess && !segs.empty()) err = hipMemcpy(h.seg[0], segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy(e->d_inst + id, &h, sizeof(Inst), hipMemcpyHostToDevice);
 	if (err == hipSuccess && stored) { /* host records -> SoA, on the device */
@@ -1877,8 +1952,13 @@ extern "C" int sampo_engine_export(sampo_engine *e, uint32_t slot, sampo_layout 
 	sampo_agent *d = nullptr;
 	HIPCHK(hipMalloc(&d, (size_t)ip.n * sizeof(sampo_agent)));
 	const uint32_t *nbz = h.seeds + (size_t)ip.nb_set * 16u + 8u;
+	Hatch hb;
+	hb.inc = ip.nb_inc;
+	memcpy(hb.thr, ip.hatch_thr, sizeof hb.thr);
+	memcpy(hb.m, ip.hatch_m, sizeof hb.m);
+	memcpy(hb.b, ip.hatch_b, sizeof hb.b);
 	hipLaunchKernelGGL(k_export, dim3((ip.n + 255u) / 256u), dim3(256), 0, e->stream, h.buf[ip.cur], h.P, h.seg[par],
-	                   ip.nseg, ip.n, nbz, ip.nb_temp, d);
+	                   ip.nseg, ip.n, nbz, hb, d);
 	hipError_t err = hipGetLastError();
 	if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
 	if (err == hipSuccess) err = hipMemcpy(agents, d, (size_t)ip.n * sizeof(sampo_agent), hipMemcpyDeviceToHost);
@@ -1975,8 +2055,9 @@ extern "C" int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_
 	uint64_t x = seed ^ 0x504f50u;
 	std::vector<sampo_agent> by[4];
 	const int stage_of[4] = {S_EGG, S_LARVA, S_PUPA, S_IMM};
-	static const float rates[32] = SM_DEV_RATES_INIT;
-	const float dev = rates[sm_dev_index(t0)];
+	sampo_species sp; /* the stages' delays of include/properties.h */
+	sampo_species_default(&sp);
+	const float dev = sm_dev_rate(&sp, t0);
 	for (uint32_t i = 0; i < n0; ++i) {
 		double us = sm64_unit(&x);
 		int st = us < 0.3 ? 0 : us < 0.6 ? 1 : us < 0.7 ? 2 : 3;
@@ -1986,7 +2067,7 @@ extern "C" int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_
 		double u = sm64_unit(&x);
 		switch (st) {
 		case 0: {
-			a.delay = sm_egg_delay(t0, (float)sm64_unit(&x));
+			a.delay = sm_egg_delay(&sp, t0, (float)sm64_unit(&x));
 			float h = __builtin_floorf((float)u * a.delay);
 			a.ageInHours = a.hoursInState = h > 0.0f ? h : 0.0f;
 			break;
@@ -2000,14 +2081,14 @@ extern "C" int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_
 			break;
 		}
 		case 2: {
-			a.delay = (float)(-0.923 * (double)t0 + 60.923);
+			a.delay = sm_pupa_delay(&sp, t0);
 			float h = __builtin_floorf((float)u * a.delay);
 			a.hoursInState = h > 0.0f ? h : 0.0f;
 			a.ageInHours = 264.0f + a.hoursInState;
 			break;
 		}
 		default: {
-			a.delay = -2.667f * t0 + 120.0f;
+			a.delay = sm_linear(sp.immature_delay, t0);
 			float h = __builtin_floorf((float)u * a.delay);
 			a.hoursInState = a.ageInHours = h > 0.0f ? h : 0.0f;
 			break;

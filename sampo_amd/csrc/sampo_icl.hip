@@ -67,67 +67,67 @@ template <typename T> T val_arg(const Arg &a, const char *kernel, int i)
 	return v;
 }
 
-typedef void (*launcher)(dim3 grid, dim3 block, hipStream_t s, const Arg *a);
+typedef void (*launcher)(dim3 grid, dim3 block, hipStream_t s, const Arg *a, const sampo_species *sp);
 
 #define B(T, i) ((T *)buf_arg(a[i], name, i))
 #define V(T, i) (val_arg<T>(a[i], name, i))
 
-void l_initAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_initAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "initAgents";
 	hipLaunchKernelGGL(initAgents, g, b, 0, s, B(IclAgent, 0), B(IclAgentAge, 1), B(IclAgentState, 2),
-	                   B(const IclEnb, 3), V(float, 4), B(const IclSeeds, 5));
+	                   B(const IclEnb, 3), V(float, 4), B(const IclSeeds, 5), sp);
 }
-void l_killAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_killAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "killAgents";
 	hipLaunchKernelGGL(killAgents, g, b, 0, s, B(IclAgentAge, 0), B(IclAgentState, 1), B(const IclPopulation, 2),
 	                   B(IclEnb, 3), B(IclBnc, 4), B(const IclSeeds, 5), V(float, 6));
 }
-void l_killAgentsStaged(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_killAgentsStaged(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "killAgentsStaged";
 	hipLaunchKernelGGL(killAgentsStaged, g, b, 0, s, B(IclAgentAge, 0), B(IclAgentState, 1),
 	                   B(const IclPopulation, 2), B(IclEnb, 3), B(IclBnc, 4), B(const IclSeeds, 5), V(float, 6),
 	                   V(float, 7), V(float, 8), V(float, 9));
 }
-void l_updateAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_updateAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "updateAgents";
 	hipLaunchKernelGGL(updateAgents, g, b, 0, s, B(IclAgent, 0), B(IclAgentAge, 1), B(IclAgentState, 2),
 	                   B(const IclPopulation, 3), V(IclEnvironment, 4), V(float, 5), B(IclBnc, 6), B(IclEnb, 7),
-	                   B(const IclSeeds, 8), V(float, 9), V(uint32_t, 10));
+	                   B(const IclSeeds, 8), V(float, 9), V(uint32_t, 10), sp);
 }
-void l_calcL1dePerGroup(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_calcL1dePerGroup(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "calcL1dePerGroup";
 	hipLaunchKernelGGL(calcL1dePerGroup, g, b, 0, s, B(const IclAgentAge, 0), B(const IclPopulation, 1),
 	                   B(uint32_t, 2));
 }
-void l_calcL1deTotal(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_calcL1deTotal(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "calcL1deTotal";
 	hipLaunchKernelGGL(calcL1deTotal, g, b, 0, s, B(IclPopulation, 0), B(const uint32_t, 1), V(uint32_t, 2));
 }
-void l_calcFemalesPerGroup(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_calcFemalesPerGroup(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "calcFemalesPerGroup";
 	hipLaunchKernelGGL(calcFemalesPerGroup, g, b, 0, s, B(const IclAgentAge, 0), B(const IclPopulation, 1),
 	                   B(uint32_t, 2));
 }
-void l_calcFemalesTotal(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_calcFemalesTotal(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "calcFemalesTotal";
 	hipLaunchKernelGGL(calcFemalesTotal, g, b, 0, s, B(IclPopulation, 0), B(const uint32_t, 1));
 }
-void l_preScan(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_preScan(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "preScan";
 	hipLaunchKernelGGL(preScan, g, b, 0, s, B(const IclAgentState, 0), B(const IclPopulation, 1), B(int32_t, 2),
 	                   B(uint32_t, 3), V(uint32_t, 4), V(uint32_t, 5), V(uint32_t, 6), V(uint32_t, 7),
 	                   V(int32_t, 8));
 }
-void l_oldToNewAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_oldToNewAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "oldToNewAgents";
 	hipLaunchKernelGGL(oldToNewAgents, g, b, 0, s, B(const IclAgent, 0), B(const IclAgentAge, 1),
@@ -135,13 +135,13 @@ void l_oldToNewAgents(dim3 g, dim3 b, hipStream_t s, const Arg *a)
 	                   B(const int32_t, 6), B(const int32_t, 7), B(const int32_t, 8), B(IclPopulation, 9),
 	                   B(const IclEnb, 10));
 }
-void l_ageHistogram(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_ageHistogram(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "ageHistogram";
 	hipLaunchKernelGGL(ageHistogram, g, b, 0, s, B(const IclAgentAge, 0), B(const IclPopulation, 1),
 	                   B(uint32_t, 2));
 }
-void l_serialScan(dim3 g, dim3 b, hipStream_t s, const Arg *a)
+void l_serialScan(dim3 g, dim3 b, hipStream_t s, const Arg *a, const sampo_species *sp)
 {
 	const char *name = "serialScan";
 	hipLaunchKernelGGL(serialScan, g, b, 0, s, B(const IclAgentState, 0), B(int32_t, 1), B(const IclPopulation, 2),
@@ -395,18 +395,50 @@ extern "C" void icl_release_buffers(cl_uint num, ...)
 
 /* ------------------------------------------------------------- kernels */
 
+/* The species table of a build-options string: -DSPECIES=<x> as the
+ * reference passes it (src/abms.c:565-568); see icl_create_kernel. */
+static void species_of(const char *opts, sampo_species *sp)
+{
+	sampo_species_default(sp);
+	if (!opts) return;
+	std::vector<std::string> inc, toks;
+	for (const char *p = opts; *p;) {
+		while (*p == ' ' || *p == '\t') ++p;
+		const char *q = p;
+		while (*q && *q != ' ' && *q != '\t') ++q;
+		if (q > p) toks.emplace_back(p, q);
+		p = q;
+	}
+	std::string name;
+	for (const std::string &t : toks) {
+		if (t.rfind("-I", 0) == 0 && t.size() > 2) inc.push_back(t.substr(2));
+		if (t.rfind("-DSPECIES=", 0) == 0) name = t.substr(10);
+	}
+	const std::string dflt = "properties.h";
+	if (name.empty() ||
+	    (name.size() >= dflt.size() && name.compare(name.size() - dflt.size(), dflt.size(), dflt) == 0))
+		return;
+	int rc = sampo_species_read(name.c_str(), sp);
+	for (size_t i = 0; rc && i < inc.size(); ++i) rc = sampo_species_read((inc[i] + "/" + name).c_str(), sp);
+	if (rc) die("icl_create_kernel: cannot read a valid species file", name.c_str());
+}
+
 extern "C" icl_kernel *icl_create_kernel(icl_device *dev, const char *file_name, const char *kernel_name,
                                          const char *build_options, icl_create_kernel_flag flag)
 {
-	(void)build_options;
 	(void)flag;
 	const char *fb = base_name(file_name ? file_name : "");
 	for (int i = 0; i < k_nreg; ++i) {
 		if (strcmp(k_registry[i].name, kernel_name) != 0) continue;
 		if (*fb && strcmp(fb, k_registry[i].file) != 0) die("icl_create_kernel: kernel not in that file", kernel_name);
+		sampo_species sp;
+		species_of(build_options, &sp);
 		icl_kernel *k = (icl_kernel *)malloc(sizeof(icl_kernel));
 		k->id = i;
 		k->dev = dev;
+		HIP_OR_DIE(hipSetDevice(dev->hip_device));
+		HIP_OR_DIE(hipMalloc(&k->species, sizeof(sampo_species)));
+		HIP_OR_DIE(hipMemcpy(k->species, &sp, sizeof(sp), hipMemcpyHostToDevice));
 		return k;
 	}
 	die("icl_create_kernel: no such kernel", kernel_name);
@@ -436,12 +468,18 @@ extern "C" void icl_run_kernel(const icl_kernel *kernel, cl_uint work_dim, const
 	hipStream_t s = stream_of(kernel->dev);
 	wait_on(wait_event, s);
 	record_start(event, s);
-	e.fn(dim3((unsigned)(global / local)), dim3((unsigned)local), s, a);
+	e.fn(dim3((unsigned)(global / local)), dim3((unsigned)local), s, a, (const sampo_species *)kernel->species);
 	HIP_OR_DIE(hipGetLastError());
 	record_stop(event, s);
 }
 
-extern "C" void icl_release_kernel(icl_kernel *kernel) { free(kernel); }
+extern "C" void icl_release_kernel(icl_kernel *kernel)
+{
+	if (!kernel) return;
+	(void)hipStreamSynchronize(stream_of(kernel->dev));
+	HIP_OR_DIE(hipFree(kernel->species));
+	free(kernel);
+}
 
 /* ------------------------------------------------------------- events, timers */
 

@@ -15,8 +15,6 @@
 
 #define LOCAL 64u /* LOCAL_SIZE, include/agent.h:16 */
 
-__constant__ float c_rates[32] = SM_DEV_RATES_INIT;
-
 __device__ static inline uint32_t gid1() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
 __device__ static inline float taus(const IclSeeds *s, uint32_t g) { return sc_hybrid_taus(s->z1, s->z2, s->z3, s->z4, g); }
@@ -31,11 +29,9 @@ __device__ static inline bool in_range(const IclPopulation *p, uint32_t g)
 
 __device__ static inline float hmr(float dmr) { return 1.0f - sc_powf(1.0f - dmr, 1.0f / 24.0f); }
 
-__device__ static inline int night(uint32_t h) { return h <= 6u || h >= 18u; }
-
-/* kernel/initAgents.cl:26-65 */
+/* kernel/initAgents.cl:26-65; sp is the species table (the -DSPECIES header) */
 __global__ void initAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState *states, const IclEnb *enb,
-                           float temperature, const IclSeeds *seeds)
+                           float temperature, const IclSeeds *seeds, const sampo_species *sp)
 {
 	uint32_t id = gid1();
 	if (id >= enb->newEggs) return;
@@ -52,14 +48,7 @@ __global__ void initAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState *s
 	ag.availableEggs = 0u;
 	ag.cycleLength = 0u;
 	ag.aux = 0u;
-	float p = taus(&seeds[2], id);
-	float m, b;
-	if (p < 0.5f) { m = 48.0f; b = 0.0f; }
-	else if (p < 0.85f) { m = 68.5714f; b = -10.2857f; }
-	else if (p < 0.90f) { m = 480.0f; b = -360.0f; }
-	else if (p < 0.94f) { m = 600.00f; b = -468.0f; }
-	else { m = 2400.0f; b = -2160.0f; }
-	ag.delay = (-0.923f * temperature + 60.923f) + (float)sc_f2u(m * p + b + 0.5f);
+	ag.delay = sm_egg_delay(sp, temperature, taus(&seeds[2], id));
 	ag.numEggBatches = 0u;
 	agents[id] = ag;
 }
@@ -121,21 +110,21 @@ __global__ void killAgentsStaged(IclAgentAge *agents, IclAgentState *states, con
 }
 
 /* include/properties.h:257-266 */
-__device__ __attribute__((noinline)) static uint32_t gen_eggs(float u0, float u1, uint32_t nb)
+__device__ __attribute__((noinline)) static uint32_t gen_eggs(float u0, float u1, uint32_t nb, const sampo_species *sp)
 {
-	uint32_t eggs = sc_f2u(sc_box_muller(u0, u1, 170.0f, 30.0f) + 0.5f);
-	return sc_f2u((float)eggs * sc_powf(0.8f, (float)nb) + 0.5f);
+	return sm_egg_batch(u0, u1, sp->egg_batch[0], sp->egg_batch[1], sp->egg_batch[2], nb);
 }
 
-__device__ __attribute__((noinline)) static float larva_delay(float u0, float u1)
+/* include/properties.h:87-91 */
+__device__ __attribute__((noinline)) static float larva_delay(float u0, float u1, const sampo_species *sp)
 {
-	return sc_box_muller(u0, u1, 1.0f, 0.1f);
+	return sc_box_muller(u0, u1, sp->larva_delay[0], sp->larva_delay[1]);
 }
 
 /* kernel/updateAgents.cl:39-270 */
 __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState *states, const IclPopulation *pop,
                              IclEnvironment env, float t, IclBnc *bnc, IclEnb *enb, const IclSeeds *seeds,
-                             float dt, uint32_t h)
+                             float dt, uint32_t h, const sampo_species *sp)
 {
 	uint32_t g = gid1();
 	if (!in_range(pop, g)) return;
@@ -152,10 +141,10 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 		ae.csd = (float)((double)ae.csd + (double)cond * x);
 	}
 	if (g < pop->r[0].end) {
-		if (ae.hoursInState >= a.delay) {
+		if (ae.hoursInState >= a.delay && sm_open(sp, 0, h)) {
 			ae.ageInHours = 24.0f;
 			ae.hoursInState = 0.0f;
-			a.delay = larva_delay(taus(&seeds[1], g), taus(&seeds[2], g));
+			a.delay = larva_delay(taus(&seeds[1], g), taus(&seeds[2], g), sp);
 			st.state = ICL_LARVA;
 			agents[g] = a;
 			states[g] = st;
@@ -169,11 +158,11 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 			states[g] = st;
 			return;
 		}
-		float cl = sc_float(a.aux) + c_rates[sm_dev_index(t)];
+		float cl = sc_float(a.aux) + sm_dev_rate(sp, t);
 		a.aux = sc_bits_f(cl);
-		if (cl >= a.delay && night(h)) {
+		if (cl >= a.delay && sm_open(sp, 1, h)) {
 			ae.hoursInState = 0.0f;
-			a.delay = (float)(-0.923 * (double)t + 60.923);
+			a.delay = sm_pupa_delay(sp, t);
 			st.state = ICL_PUPA;
 			states[g] = st;
 		}
@@ -182,10 +171,10 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 		return;
 	}
 	if (g < pop->r[2].end) {
-		if (ae.hoursInState >= a.delay && night(h)) {
+		if (ae.hoursInState >= a.delay && sm_open(sp, 2, h)) {
 			ae.ageInHours = 0.0f;
 			ae.hoursInState = 0.0f;
-			a.delay = -2.667f * t + 120.0f;
+			a.delay = sm_linear(sp->immature_delay, t);
 			st.state = ICL_IMMATURE;
 			agents[g] = a;
 			states[g] = st;
@@ -194,8 +183,8 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 		return;
 	}
 	if (g < pop->r[3].end) {
-		if (ae.hoursInState >= a.delay) {
-			a.delay = 1.0f;
+		if (ae.hoursInState >= a.delay && sm_open(sp, 3, h)) {
+			a.delay = sp->mate_seeking_delay;
 			ae.hoursInState = 0.0f;
 			st.state = ICL_MATESEEKING;
 			agents[g] = a;
@@ -205,9 +194,9 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 		return;
 	}
 	if (g < pop->r[4].end) {
-		if (ae.hoursInState >= a.delay && ae.isFemale && h == 18u) {
+		if (ae.hoursInState >= a.delay && ae.isFemale && sm_open(sp, 4, h)) {
 			ae.hoursInState = 0.0f;
-			a.delay = 0.0f;
+			a.delay = sp->bms_delay;
 			st.state = ICL_BMS;
 			states[g] = st;
 			agents[g] = a;
@@ -217,19 +206,20 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 	}
 	if (g < pop->r[5].end) {
 		++a.cycleLength;
-		if (ae.hoursInState >= a.delay && night(h) && taus(&seeds[1], g) <= env.bloodmealSuccess) {
-			float itn = env.ITNValue * 1.0f * 1.0f;
+		if (ae.hoursInState >= a.delay && sm_open(sp, 5, h) && taus(&seeds[1], g) <= env.bloodmealSuccess) {
+			float itn = env.ITNValue * sp->bite_human * sp->bite_indoor;
 			if (itn > 0.0f && taus(&seeds[2], g) <= itn) {
 				st.dead = 1u;
 				states[g] = st;
 				return;
 			}
-			a.humanBloodmealCount += 1u;
+			uint32_t human = sp->bite_human < 1.0f ? (taus(&seeds[3], g) <= sp->bite_human) : 1u;
+			a.humanBloodmealCount += human;
 			atomicAdd(&bnc->numBites, 1u);
 			ae.hoursInState = 0.0f;
-			a.delay = -1.231f * t + 77.231f;
+			a.delay = sm_linear(sp->bmd_delay, t);
 			st.state = ICL_BMD;
-			if (ae.csd >= 1.0f) atomicAdd(&bnc->numInfectBites, 1u);
+			if (ae.csd >= 1.0f && human) atomicAdd(&bnc->numInfectBites, 1u);
 			states[g] = st;
 		}
 		agents[g] = a;
@@ -237,16 +227,16 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 		return;
 	}
 	if (g < pop->r[6].end) {
-		float irs = env.IRSValue * 1.0f;
+		float irs = env.IRSValue * sp->rest_indoor;
 		if (irs > 0.0f && taus(&seeds[2], g) <= irs) {
 			st.dead = 1u;
 			states[g] = st;
 			return;
 		}
 		++a.cycleLength;
-		if (ae.hoursInState >= a.delay) {
+		if (ae.hoursInState >= a.delay && sm_open(sp, 6, h)) {
 			if (a.availableEggs == 0u && ae.isFemale) {
-				a.availableEggs = gen_eggs(taus(&seeds[1], g), taus(&seeds[2], g), a.numEggBatches);
+				a.availableEggs = gen_eggs(taus(&seeds[1], g), taus(&seeds[2], g), a.numEggBatches, sp);
 				a.numEggBatches += 1u;
 			}
 			ae.hoursInState = 0.0f;
@@ -259,7 +249,7 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 		return;
 	}
 	++a.cycleLength;
-	if (night(h) && taus(&seeds[1], g) < 0.25f) {
+	if (sm_open(sp, 7, h) && taus(&seeds[1], g) < sp->lay_probability) {
 		if (env.oviTrapValue > 0.0f && taus(&seeds[2], g) <= env.oviTrapValue) {
 			st.dead = 1u;
 			a.availableEggs = 0u;
@@ -278,7 +268,7 @@ __global__ void updateAgents(IclAgent *agents, IclAgentAge *ages, IclAgentState 
 			atomicAdd(&bnc->sumCycles, a.cycleLength);
 			a.cycleLength = 0u;
 			ae.hoursInState = 0.0f;
-			a.delay = 0.0f;
+			a.delay = sp->bms_delay;
 			st.state = ICL_BMS;
 			states[g] = st;
 		}

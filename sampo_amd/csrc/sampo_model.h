@@ -35,38 +35,54 @@ enum { S_EGG = 0, S_LARVA, S_PUPA, S_IMM, S_MS, S_BMS, S_BMD, S_GRAVID, S_DEAD }
 
 #define SAMPO_PAD 64u  /* LOCAL_SIZE of the reference layout (include/agent.h:16) */
 
-/* devRates, include/properties.h:94-98: hourly larval development at
- * 12, 13, ..., 43 degC; the one initializer every table in the library uses */
-#define SM_DEV_RATES_INIT { \
-	0.000175044f, 0.000549094f, 0.00109031f, 0.001494158f, 0.001752487f, 0.001962866f, \
-	0.002171238f, 0.002392989f, 0.002633689f, 0.00289619f, 0.003182685f, 0.003495287f, \
-	0.003836204f, 0.004207789f, 0.004612566f, 0.005053243f, 0.005532729f, 0.006054144f, \
-	0.006620836f, 0.007236393f, 0.007904657f, 0.008629745f, 0.009416051f, 0.010267388f, \
-	0.011113704f, 0.011960019f, 0.012806335f, 0.010401163f, 0.007995991f, 0.00033845f, \
-	0.00000533086f, 0.0000000840986f}
+/* The species functions of include/properties.h, evaluated from a species
+ * table (sampo_species) in the arithmetic the header uses. */
 
-/* devRates index of larvaDevelopment (properties.h:105-109): T clamped to
- * [12, 48], truncated, minus 12, and clamped to the 32-entry table */
-SC_FN int sm_dev_index(float t)
+/* larvaDevelopment (properties.h:105-109): T clamped to the table's range,
+ * truncated, minus the range start; the index is clamped to the 32 entries
+ * (the reference reads past the table for T >= 44 degC) */
+SC_FN int sm_dev_index(const sampo_species *sp, float t)
 {
-	float tc = sc_minf(sc_maxf(t, 12.0f), 48.0f);
-	int ix = (int)tc - 12;
+	float tc = sc_minf(sc_maxf(t, sp->dev_temp_range[0]), sp->dev_temp_range[1]);
+	int ix = (int)tc - (int)sp->dev_temp_range[0];
 	return ix > 31 ? 31 : ix;
 }
 
-/* properties.h:29-31 and 39-60 */
-SC_FN float sm_hatching(float p)
+SC_FN float sm_dev_rate(const sampo_species *sp, float t) { return sp->dev_rates[sm_dev_index(sp, t)]; }
+
+/* eggsHatching (properties.h:39-60): the first band whose threshold exceeds
+ * p, rounded to an integer number of hours */
+SC_FN float sm_hatching(const float *thr, const float *m, const float *b, float p)
 {
-	float m, b;
-	if (p < 0.5f) { m = 48.0f; b = 0.0f; }
-	else if (p < 0.85f) { m = 68.5714f; b = -10.2857f; }
-	else if (p < 0.90f) { m = 480.0f; b = -360.0f; }
-	else if (p < 0.94f) { m = 600.00f; b = -468.0f; }
-	else { m = 2400.0f; b = -2160.0f; }
-	return (float)sc_f2u(m * p + b + 0.5f);
+	int i = 0;
+	while (i < 4 && !(p < thr[i])) ++i;
+	return (float)sc_f2u(m[i] * p + b[i] + 0.5f);
 }
 
-SC_FN float sm_egg_delay(float t, float p) { return (-0.923f * t + 60.923f) + sm_hatching(p); }
+SC_FN float sm_linear(const float *ab, float t) { return ab[0] * t + ab[1]; }
+
+/* initAgents.cl:57: eggsIncubation(T) + eggsHatching(p) */
+SC_FN float sm_egg_delay(const sampo_species *sp, float t, float p)
+{
+	return sm_linear(sp->egg_incubation, t) +
+	       sm_hatching(sp->hatch_threshold, sp->hatch_slope, sp->hatch_offset, p);
+}
+
+/* pupaDelay (properties.h:133-135): binary64, rounded once */
+SC_FN float sm_pupa_delay(const sampo_species *sp, float t)
+{
+	return (float)(sp->pupa_delay[0] * (double)t + sp->pupa_delay[1]);
+}
+
+/* the transition-time predicate of state s (7: egg laying) at hour h */
+SC_FN int sm_open(const sampo_species *sp, int s, uint32_t h) { return (int)((sp->window[s] >> h) & 1u); }
+
+/* generateEggs (properties.h:257-266) for batch nb from the pair (u0, u1) */
+SC_FN uint32_t sm_egg_batch(float u0, float u1, float mean, float sd, float decay, uint32_t nb)
+{
+	uint32_t eggs = sc_f2u(sc_box_muller(u0, u1, mean, sd) + 0.5f);
+	return sc_f2u((float)eggs * sc_powf(decay, (float)nb) + 0.5f);
+}
 
 /* killAgents.cl:91-93: HMR = 1 - (1 - DMR)^(1/24) */
 SC_FN float sm_hmr(float dmr) { return 1.0f - sc_powf(1.0f - dmr, 1.0f / 24.0f); }

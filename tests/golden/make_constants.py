@@ -21,6 +21,56 @@ def grab(path, pattern, flags=0):
     return m, line
 
 
+def hour_mask(expr):
+    """The hours 0..23 at which a transition-time predicate holds, from its
+    return expression: `true`, or terms `worldTime <op> <n>[u]` joined by ||."""
+    expr = expr.strip()
+    while expr.startswith("(") and expr.endswith(")"):
+        expr = expr[1:-1].strip()
+    if expr == "true":
+        return (1 << 24) - 1
+    ops = {"<=": lambda h, n: h <= n, ">=": lambda h, n: h >= n, "==": lambda h, n: h == n,
+           "<": lambda h, n: h < n, ">": lambda h, n: h > n}
+    mask = 0
+    for term in expr.split("||"):
+        m = re.fullmatch(r"\s*worldTime\s*(<=|>=|==|<|>)\s*(\d+)u?\s*", term)
+        if not m:
+            raise SystemExit(f"unrecognised transition predicate: {expr}")
+        mask |= sum(1 << h for h in range(24) if ops[m.group(1)](h, int(m.group(2))))
+    return mask
+
+
+def species(props, update):
+    """The members of the species table that the header fixes beyond the
+    delay coefficients above: transition-hour windows, bite/rest
+    probabilities, the larvaDevelopment clamp, constant delays, the egg-batch
+    decay, and the egg-laying probability of updateAgents.cl."""
+    out = {}
+    text = open(props).read()
+    preds = ["eggsTransitionTime", "larvaeTransitionTime", "pupaeTransitionTime", "immaturesTransitionTime",
+             "mateSeekingsTransitionTime", "bloodMealSeekingsTransitionTime", "bloodMealDigestingsTransitionTime",
+             "gravidsEggLayTime"]
+    out["window"] = []
+    for fn in preds:
+        m, line = grab(props, r"bool " + fn + r"\(UINT worldTime\) \{\s*return\s*([^;]*);", re.S)
+        out["window"].append({"fn": fn, "mask": hour_mask(m.group(1)), "source": f"include/properties.h:{line}"})
+    for d in ("BITE_HUMAN_PROBABILITY", "BITE_INDOOR_PROBABILITY", "REST_INDOOR_PROBABILITY"):
+        m, line = grab(props, r"#define " + d + r" ([0-9.]+)f")
+        out[d] = {"value": float(m.group(1)), "source": f"include/properties.h:{line}"}
+    m, line = grab(props, r"min\(max\(temperature, ([0-9.]+)f\), ([0-9.]+)f\)")
+    m2, _ = grab(props, r"devRates\[\(\(int\)clampedTemp\) - (\d+)\]")
+    out["larvaDevelopment"] = {"clamp": [float(m.group(1)), float(m.group(2))], "base": int(m2.group(1)),
+                               "source": f"include/properties.h:{line}"}
+    for fn in ("mateSeekingDelay", "bloodMealSeekingDelay"):
+        m, line = grab(props, r"REAL " + fn + r"\(Temperature temperature\) \{\s*return ([0-9.]+)f;", re.S)
+        out[fn] = {"value": float(m.group(1)), "source": f"include/properties.h:{line}"}
+    m, line = grab(props, r"pow\(([0-9]*\.[0-9]+)f,")
+    out["eggBatchDecay"] = {"value": float(m.group(1)), "source": f"include/properties.h:{line}"}
+    m, line = grab(update, r"shouldLayEggs = HybridTaus\(&seeds\[1\]\) < ([0-9.]+)f")
+    out["layProbability"] = {"value": float(m.group(1)), "source": f"kernel/updateAgents.cl:{line}"}
+    return out
+
+
 def main(ref):
     props = os.path.join(ref, "include", "properties.h")
     out = {}
@@ -61,6 +111,7 @@ def main(ref):
                    "initialAgentCount": re.search(r"UINT initialAgentCount = ([^;]*);", txt).group(1),
                    "carryingCapacity": re.search(r"REAL carryingCapacity = ([^;]*);", txt).group(1),
                    "maxSteps": int(re.search(r"long long maxSteps = (\d+);", txt).group(1))}
+    out["species"] = species(props, os.path.join(ref, "kernel", "updateAgents.cl"))
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "constants.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=1)

@@ -83,22 +83,15 @@ def test_headers_declare_the_reference_boundary():
         assert n in names, n
 
 
-def test_library_dev_rates_table_is_the_reference_table():
-    text = open(os.path.join(CSRC, "sampo_model.h")).read()
-    m = re.search(r"#define SM_DEV_RATES_INIT \{(.*?)\}", text, re.S)
-    vals = np.float32([float(x.rstrip("f")) for x in re.findall(r"[0-9.eE+-]+f", m.group(1))])
-    assert np.array_equal(vals, np.float32(CONST["devRates"]["values"]))
-    # every device table in the library is initialised from it
-    for f in ("sampo_engine.hip", "sampo_icl_kernels.hip"):
+def test_species_constants_live_only_in_the_species_table():
+    """include/properties.h's numbers are written once, in sampo_species.c
+    (checked against the reference in test_species.py); every kernel reads
+    them from a species table."""
+    for f in os.listdir(CSRC):
+        if f == "sampo_species.c" or not f.endswith((".hip", ".h", ".c")):
+            continue
         src = open(os.path.join(CSRC, f)).read()
-        assert "0.000175044f" not in src and "SM_DEV_RATES_INIT" in src
+        for lit in ("0.000175044", "68.5714", "-0.923", "2.667", "77.231", "170.0f", "0.25f"):
+            assert lit not in src, (f, lit)
 
 
-def test_library_hatching_coefficients_are_the_reference_coefficients():
-    text = open(os.path.join(CSRC, "sampo_model.h")).read()
-    body = re.search(r"float sm_hatching\(float p\)\s*\{(.*?)return", text, re.S).group(1)
-    thr = [float(x) for x in re.findall(r"p < ([0-9.]+)f", body)]
-    ms = [float(x) for x in re.findall(r"m = (-?[0-9.]+)f", body)]
-    bs = [float(x) for x in re.findall(r"b = (-?[0-9.]+)f", body)]
-    h = CONST["eggsHatching"]
-    assert thr == h["thresholds"] and ms == h["m"] and bs == h["b"]
