@@ -29,7 +29,10 @@ Besides the JSON line's throughput, the single-run line carries:
                fused-pass launch / that launch's duration from HIP events
                recorded around every pass of the timed region;
   cpu_baseline the C restatement of the reference (oracle/), OpenMP over all
-               host cores, on a bounded sample of the same workload.
+               host cores, on a bounded sample of the same workload;
+  compaction_scan  the compaction index as a separate step (per-state ranks,
+               include/sampo_compact.h) on as many synthetic keys as the
+               workload's capacity: 5 bytes per agent / duration (HIP events).
 """
 import argparse
 import json
@@ -260,6 +263,10 @@ def main():
         achieved = per_launch / (pass_ms / n_iter / 1e3) / 1e9
         copy_bytes = 1 << 30
         copy_ms = S.stream_copy_ms(copy_bytes)
+    # the compaction index as a separate step (include/sampo_compact.h): a
+    # 1-byte key in and a 4-byte rank out per agent, at the workload's capacity
+    scan_n = int(w["max_agents"])
+    scan_ms = S.compact_time(scan_n, iters=20, seed=w["seed"])
     value = steps_done / dt
     out = {
         "metric": METRIC, "value": value, "unit": "agent-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -277,6 +284,10 @@ def main():
                      "mean_launch_ms": pass_ms / n_iter, "stream_copy_GBs": 2 * copy_bytes / (copy_ms / 1e3) / 1e9},
         "gpu_wall_ms_events": wall_ms,
         "admit_ms": admit_ms,
+        "compaction_scan": {"agents": scan_n, "bytes_per_agent": 5, "ms": scan_ms,
+                            "GBs": 5 * scan_n / (scan_ms / 1e3) / 1e9,
+                            "frac": 5 * scan_n / (scan_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                            "kernels": "k_count, k_scan_local, k_scan_top, k_rank"},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_days, args.cpu_scale)

@@ -118,6 +118,11 @@ def lib():
         L.sampo_seed_stream.restype = None
         L.sampo_gen_temperature.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.c_float, C.c_float, vp]
         L.sampo_gen_population.argtypes = [C.c_uint64, C.c_uint32, C.c_float, C.POINTER(Layout), vp]
+        L.sampo_compact_workspace_bytes.argtypes = [C.c_uint64]
+        L.sampo_compact_workspace_bytes.restype = C.c_uint64
+        L.sampo_compact_index.argtypes = [vp, C.c_uint64, vp, vp, vp, C.c_uint64, vp]
+        L.sampo_compact_index_host.argtypes = [C.c_int, vp, C.c_uint64, vp, vp]
+        L.sampo_compact_time.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_float)]
         L.sampo_species_default.argtypes = [vp]
         L.sampo_species_default.restype = None
         L.sampo_species_check.argtypes = [vp]
@@ -169,6 +174,37 @@ def device_memory(device=0):
 def make_env(bloodmeal=0.25, itn=0.0, irs=0.0, cc=160000.0, larvicide=0.0, ovitrap=0.0):
     return np.array([(bloodmeal, itn, irs, cc, larvicide, ovitrap)],
                     dtype=[(n, "<f4") for n, _ in Env._fields_])
+
+
+COMPACT_NONE = 0xFFFFFFFF
+
+
+def compact_index(keys, device=0):
+    """Per-key ranks of the compaction step (sampo_compact_index) on host
+    data: dest[i] = number of earlier keys equal to keys[i] for keys 0..7,
+    COMPACT_NONE otherwise; counts[k] = number of keys equal to k."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint8)
+    dest = np.empty(keys.shape[0], np.uint32)
+    counts = np.zeros(8, np.uint32)
+    _check(lib().sampo_compact_index_host(device, keys.ctypes.data, keys.shape[0], dest.ctypes.data,
+                                          counts.ctypes.data))
+    return dest, counts
+
+
+def compact_index_device(keys_ptr, n, dest_ptr, counts_ptr, workspace_ptr, workspace_bytes, stream=None):
+    """sampo_compact_index on device pointers (integers), asynchronous."""
+    _check(lib().sampo_compact_index(keys_ptr, n, dest_ptr, counts_ptr, workspace_ptr, workspace_bytes, stream))
+
+
+def compact_workspace_bytes(n):
+    return int(lib().sampo_compact_workspace_bytes(n))
+
+
+def compact_time(n, iters=20, seed=1, device=0):
+    """Milliseconds per sampo_compact_index call on n device-generated keys."""
+    ms = C.c_float()
+    _check(lib().sampo_compact_time(device, n, iters, seed, C.byref(ms)))
+    return ms.value
 
 
 def default_species():

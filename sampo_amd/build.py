@@ -27,43 +27,42 @@ HIP_FLAGS = [
 ]
 
 TARGETS = {
-    "libsampo_engine.so": ["sampo_engine.hip", "sampo_species.c"],
+    "libsampo_engine.so": ["sampo_engine.hip", "sampo_compact.hip", "sampo_species.c"],
     "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c", "sampo_report.c", "sampo_species.c"],
 }
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 
+def _run(cmd, what, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        sys.stderr.write(msgs + "\n")
+        raise RuntimeError(f"build failed: {what}")
+    if verbose and msgs:
+        print(msgs)
+
+
 def build(verbose=False):
+    """Compile every source to an object (gcc for host C, hipcc for HIP) and
+    link each library with hipcc."""
     out = []
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
     for lib, srcs in TARGETS.items():
-        dst = os.path.join(HERE, lib)
         objs = []
         for s in srcs:
-            if s.endswith(".c"):  # host C: gcc, canonical flags
-                obj = os.path.join(HERE, "_build", s[:-2] + ".o")
-                os.makedirs(os.path.dirname(obj), exist_ok=True)
-                subprocess.run(["gcc", *C_FLAGS, "-c", "-o", obj, os.path.join(CSRC, s)], check=True)
-                objs.append(obj)
-        hip_srcs = [os.path.join(CSRC, s) for s in srcs if not s.endswith(".c")]
-        cmd = ["hipcc", *HIP_FLAGS, "-o", dst, *hip_srcs]
-        if objs:  # hipcc reads every input as HIP source: link host objects in a second step
-            hobj = os.path.join(HERE, "_build", lib + ".hip.o")
-            cmd = ["hipcc", *[f for f in HIP_FLAGS if f != "-shared"], "-c", "-o", hobj, *hip_srcs]
-            r = subprocess.run(cmd, capture_output=True, text=True)
-            if r.returncode != 0:
-                sys.stderr.write(r.stdout + r.stderr)
-                raise RuntimeError(f"hipcc failed for {lib}")
-            cmd = ["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", dst, hobj, *objs]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        msgs = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
-                         if "packed-fp32-ops" not in l and l.strip())
-        if r.returncode != 0:
-            sys.stderr.write(msgs + "\n")
-            raise RuntimeError(f"hipcc failed for {lib}")
-        if verbose and msgs:
-            print(msgs)
+            obj = os.path.join(bdir, s.rsplit(".", 1)[0] + ".o")
+            src = os.path.join(CSRC, s)
+            if s.endswith(".c"):
+                _run(["gcc", *C_FLAGS, "-c", "-o", obj, src], s, verbose)
+            else:
+                _run(["hipcc", *[f for f in HIP_FLAGS if f != "-shared"], "-c", "-o", obj, src], s, verbose)
+            objs.append(obj)
+        dst = os.path.join(HERE, lib)
+        _run(["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", dst, *objs], lib, verbose)
         out.append(dst)
     return out
 

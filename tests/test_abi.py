@@ -1,10 +1,6 @@
 """The C ABI as declared in include/*.h (CPU: loads the built libraries and
-checks their exports and model constants; no device calls).
-
-Also pins the library's own copies of the reference's model constants: the
-devRates initializer in sampo_model.h and the hatching coefficients of
-sm_hatching() against tests/golden/constants.json (constants read as text
-from include/properties.h).
+checks their exports; no device calls), and that the model constants of
+include/properties.h are written once, in the species table.
 """
 import ctypes
 import json
@@ -20,11 +16,11 @@ INC = os.path.join(ROOT, "include")
 CSRC = os.path.join(ROOT, "sampo_amd", "csrc")
 LIBS = {
     "sampo_engine.h": os.path.join(ROOT, "sampo_amd", "libsampo_engine.so"),
+    "sampo_compact.h": os.path.join(ROOT, "sampo_amd", "libsampo_engine.so"),
     "sampo_icl.h": os.path.join(ROOT, "sampo_amd", "libsampo_icl.so"),
     "sampo_abms.h": os.path.join(ROOT, "sampo_amd", "libsampo_icl.so"),
     "sampo_report.h": os.path.join(ROOT, "sampo_amd", "libsampo_icl.so"),
 }
-CONST = json.load(open(os.path.join(ROOT, "tests", "golden", "constants.json")))
 
 
 def declared_functions(header):
