@@ -65,6 +65,33 @@ struct Chunk {
 	uint32_t pad;
 };
 
+/* How an iteration's dense range is cut into chunks: nbig chunks of `big`
+ * agents, then chunks of `small` agents (equal chunks: nbig = 0).  With few
+ * chunks per work-group, small last chunks let the persistent work-groups
+ * of the pass finish together: one that is done early takes another small
+ * one instead of the last big one holding the whole launch. */
+struct Cut {
+	uint32_t big, small, nbig;
+};
+
+__host__ __device__ static inline uint32_t cut_start(const Cut &k, uint32_t c)
+{
+	return c <= k.nbig ? c * k.big : k.nbig * k.big + (c - k.nbig) * k.small;
+}
+
+/* the first chunk starting at or after dense index x; cut_at_or_after(k, N)
+ * is the number of chunks covering N agents */
+__host__ __device__ static inline uint32_t cut_at_or_after(const Cut &k, uint32_t x)
+{
+	const uint32_t b = k.nbig * k.big;
+	return x <= b ? (x + k.big - 1u) / k.big : k.nbig + (x - b + k.small - 1u) / k.small;
+}
+
+__host__ __device__ static inline bool cut_equal(const Cut &a, const Cut &b)
+{
+	return a.big == b.big && a.small == b.small && a.nbig == b.nbig;
+}
+
 /* Output layout.  The agents of chunk c that had state s at the start of the
  * iteration occupy the dense range [head, end) of that state within the
  * chunk; in the other buffer the same slot range receives their survivors:
@@ -90,7 +117,7 @@ struct IterState {
 	uint32_t step, active, cur, n;
 	uint32_t cnt[8], vstart[8], kstart[9];
 	uint32_t nseg, nchunk, enew, nb_set;
-	uint32_t cs;           /* agents per chunk of this iteration */
+	Cut cut;               /* chunks of this iteration */
 	uint32_t seg0_done;    /* chunk seg0 already filled by the fold */
 	uint32_t tb, h, seed_set;
 	float temp;
@@ -456,15 +483,31 @@ __global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *p
 
 /* chunk size of an iteration with N agents: at least chunk_size, and at
  * most target_chunks chunks */
-__device__ static inline uint32_t chunk_agents(const Inst *I, uint32_t N)
+/* Chunks of N agents: at least chunk_size agents each, about target_chunks
+ * of them.  When that makes chunks of 4 x chunk_size or more, the first
+ * target/2 chunks take 80 % of the agents and the rest are chunks of a
+ * quarter of their size. */
+__device__ static inline Cut chunk_cut(const Inst *I, uint32_t N)
 {
-	uint32_t cs = I->chunk_size;
-	if (I->target_chunks) {
-		uint32_t want = (uint32_t)(((uint64_t)N + I->target_chunks - 1u) / I->target_chunks);
-		want = sm_roundup(want, 256u);
-		if (want > cs) cs = want;
+	const uint32_t cs = I->chunk_size, T = I->target_chunks;
+	Cut k = {cs, cs, 0u};
+	if (!T) return k;
+	const uint32_t want = sm_roundup((uint32_t)(((uint64_t)N + T - 1u) / T), 256u);
+	if (want <= cs) return k;
+	k.big = k.small = want;
+	if (want >= 4u * cs && T >= 4u) {
+		const uint32_t nb = T / 2u;
+		k.nbig = nb;
+		k.big = sm_roundup((uint32_t)(((uint64_t)N * 4u + 5u * nb - 1u) / (5u * (uint64_t)nb)), 256u);
+		const uint32_t rest = N > nb * k.big ? N - nb * k.big : 0u;
+		const uint32_t sm = sm_roundup((uint32_t)(((uint64_t)rest + (T - nb) - 1u) / (T - nb)), 256u);
+		k.small = sm > cs ? sm : cs;
+		if (nb * (uint64_t)k.big >= N) { /* rounding left no tail: equal chunks */
+			k.big = k.small = want;
+			k.nbig = 0u;
+		}
 	}
-	return cs;
+	return k;
 }
 
 /* Exclusive scan of N values per thread over the work-group, all at once;
@@ -525,7 +568,8 @@ __device__ static inline uint32_t run_row(uint32_t sn, uint32_t kind)
 __device__ static void fold_iteration(Inst *I, uint32_t par)
 {
 	constexpr uint32_t KR = 2; /* chunks per thread held in registers */
-	__shared__ uint32_t s_sum[8], s_kst[9], s_vst[8], s_sb[8], s_tot[16], s_cs, s_C;
+	__shared__ uint32_t s_sum[8], s_kst[9], s_vst[8], s_sb[8], s_tot[16], s_C;
+	__shared__ Cut s_cut;
 	const uint32_t tid = threadIdx.x, lane = tid & 63u;
 	const IterState &ip = I->st[par];
 	IterState &np = I->st[par ^ 1u];
@@ -613,16 +657,16 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		I->l1de = s_sum[5];
 		I->females = s_sum[6];
 		I->potinf = s_sum[7];
-		const uint32_t cs = chunk_agents(I, k);
-		uint32_t nc = (uint32_t)(((uint64_t)k + cs - 1u) / cs);
-		s_cs = cs;
+		const Cut nk = chunk_cut(I, k);
+		const uint32_t nc = cut_at_or_after(nk, k);
+		s_cut = nk;
 		s_C = nc < cmax ? nc : cmax;
-		np.cs = cs;
+		np.cut = nk;
 		np.seg0_done = 1u;
 	}
 	__syncthreads();
-	const uint32_t cs = s_cs, NC = s_C;
-	const uint32_t ocs = ip.cs, on = ip.n; /* chunking of the finished iteration */
+	const Cut cut = s_cut, oc = ip.cut; /* chunks of the next and of the finished iteration */
+	const uint32_t NC = s_C, on = ip.n;
 	/* segments: the newborns, then per new state the non-empty runs of
 	 * chunks 0, 1, ...  Each new chunk's first segment is recorded as its
 	 * start is passed. */
@@ -631,15 +675,15 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 			Seg sg = {0u, 0u, PB_NEWBORN, 0u};
 			seg[0] = sg;
 		}
-		for (uint32_t c = tid; c < NC && c * cs < E; c += PREP_WG) nch[c].seg0 = 0u;
+		for (uint32_t c = tid; c < NC && cut_start(cut, c) < E; c += PREP_WG) nch[c].seg0 = 0u;
 	}
 	if (c_lo >= c_hi) return;
 	/* one run: its segment at dense k0, index idx, and the chunk starts it covers */
 	auto emit = [&](uint32_t sn, uint32_t row, uint32_t c, uint32_t n, uint32_t &k0, uint32_t &idx, uint32_t &nxc,
 	                uint32_t &nxk) {
 		const uint32_t src = row & 7u, mv = row >> 3;
-		/* the source block [max(kstart[src], c*ocs), min(kstart[src+1], (c+1)*ocs, n)) */
-		const uint32_t ck0 = c * ocs, ck1 = ck0 + ocs < on ? ck0 + ocs : on;
+		/* the source block: chunk c's part of the state-src range */
+		const uint32_t ck0 = cut_start(oc, c), ce = cut_start(oc, c + 1u), ck1 = ce < on ? ce : on;
 		const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
 		const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
 		Seg sg;
@@ -648,7 +692,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		sg.pbase = mv ? be - 1u : bh;
 		sg.vbase = s_vst[sn] - s_kst[sn] + k0;
 		seg[idx] = sg;
-		for (; nxc < NC && nxk < k0 + n; ++nxc, nxk += cs) nch[nxc].seg0 = idx;
+		for (; nxc < NC && nxk < k0 + n; ++nxc, nxk = cut_start(cut, nxc)) nch[nxc].seg0 = idx;
 		++idx;
 		k0 += n;
 	};
@@ -657,7 +701,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		for (uint32_t sn = 0; sn < 8u; ++sn) {
 			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
 			uint32_t idx = s_sb[sn] + ex[8u + sn];
-			uint32_t nxc = (k0 + cs - 1u) / cs, nxk = nxc * cs; /* next new chunk start at or after k0 */
+			uint32_t nxc = cut_at_or_after(cut, k0), nxk = cut_start(cut, nxc); /* next new chunk start at or after k0 */
 #pragma unroll
 			for (uint32_t j = 0; j < KR; ++j) {
 				if (c_lo + j >= c_hi) continue;
@@ -674,7 +718,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		for (uint32_t sn = 0; sn < 8u; ++sn) {
 			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
 			uint32_t idx = s_sb[sn] + ex[8u + sn];
-			uint32_t nxc = (k0 + cs - 1u) / cs, nxk = nxc * cs;
+			uint32_t nxc = cut_at_or_after(cut, k0), nxk = cut_start(cut, nxc);
 			for (uint32_t c = c_lo; c < c_hi; ++c)
 				for (uint32_t kind = 0; kind < runs_of(sn); ++kind) {
 					const uint32_t row = run_row(sn, kind);
@@ -758,18 +802,19 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 	}
 	__syncthreads();
 	uint32_t st = s_status;
-	/* chunks: equal runs of chunk_size agents.  Their outputs go to the same
-	 * dense slot range of the other buffer, so N <= P is all the room needed. */
+	/* chunks (chunk_cut).  Their outputs go to the same dense slot range of
+	 * the other buffer, so N <= P is all the room needed. */
 	const uint32_t N = ip.n;
-	const uint32_t cs = chunk_agents(I, N);
-	const uint32_t C = st == SAMPO_RUNNING ? (N + cs - 1u) / cs : 0u;
-	const bool have_seg0 = ip.seg0_done && ip.cs == cs;
+	const Cut cut = chunk_cut(I, N);
+	const uint32_t C = st == SAMPO_RUNNING ? cut_at_or_after(cut, N) : 0u;
+	const bool have_seg0 = ip.seg0_done && cut_equal(ip.cut, cut);
 	Chunk *ch = I->chunk[par];
 	const Seg *seg = I->seg[par];
 	const uint32_t nseg = ip.nseg;
 	for (uint32_t c = tid; c < C; c += PREP_WG) {
-		ch[c].k0 = c * cs;
-		ch[c].k1 = c * cs + cs < N ? c * cs + cs : N;
+		const uint32_t e = cut_start(cut, c + 1u);
+		ch[c].k0 = cut_start(cut, c);
+		ch[c].k1 = e < N ? e : N;
 	}
 	__syncthreads();
 	/* initial populations: segment holding each chunk's first agent, segment
@@ -778,11 +823,11 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		for (uint32_t j = tid; j < nseg && C; j += PREP_WG) {
 			uint32_t a = seg[j].k0;
 			uint32_t z = j + 1u < nseg ? seg[j + 1u].k0 : N;
-			for (uint32_t c = (a + cs - 1u) / cs; c < C && c * cs < z; ++c) ch[c].seg0 = j;
+			for (uint32_t c = cut_at_or_after(cut, a); c < C && cut_start(cut, c) < z; ++c) ch[c].seg0 = j;
 		}
 	__syncthreads();
 	if (tid == 0) {
-		ip.cs = cs;
+		ip.cut = cut;
 		ip.active = st == SAMPO_RUNNING;
 		ip.nchunk = ip.active ? C : 0u;
 		I->status = st;
