@@ -27,7 +27,8 @@ sampo_amd/batch.py (per initial-size bucket), run before the timed region.
 Besides the JSON line's throughput, the single-run line carries:
   roofline     algorithmic bytes (34 B per agent in + 34 B per agent out) per
                fused-pass launch / that launch's duration from HIP events
-               recorded around every pass of the timed region;
+               recorded around every pass of the timed region (the timed
+               iterations run as one HIP graph, the events inside it);
   cpu_baseline the C restatement of the reference (oracle/), OpenMP over all
                host cores, on a bounded sample of the same workload;
   compaction_scan  the compaction index as a separate step (per-state ranks,
@@ -247,10 +248,13 @@ def main():
         e.agent_steps(reset=True)
         first = 24 * args.warmup
         n_iter = 24 * args.steps
+        e.capture(n_iter)  # the timed iterations as one HIP graph, events around each pass (untimed)
+        e.sync()
         t0 = time.perf_counter()
-        wall_ms, pass_ms = e.time_pass(n_iter)
+        e.replay()
         e.sync()
         dt = time.perf_counter() - t0
+        wall_ms, pass_ms = e.replay_times()
         status, step, n_now = e.status(slot)
         if status not in (S.RUNNING, S.DONE) or step != first + n_iter:
             sys.exit(f"instance stopped early: status {status} at iteration {step}")

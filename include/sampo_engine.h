@@ -204,6 +204,15 @@ int sampo_engine_time(sampo_engine *e, uint32_t n, int use_graph, float *ms);
  * (synchronises). */
 int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_ms, float *pass_ms);
 
+/* Capture the next n iterations as one HIP graph, with a HIP event recorded
+ * before and after every fused pass; sampo_engine_replay() launches it
+ * (asynchronous, once per capture: the iterations are those n), and
+ * sampo_engine_replay_times() then returns the time from the first pass's
+ * start to the last pass's end and the summed pass durations (synchronises). */
+int sampo_engine_capture(sampo_engine *e, uint32_t n);
+int sampo_engine_replay(sampo_engine *e);
+int sampo_engine_replay_times(sampo_engine *e, float *wall_ms, float *pass_ms);
+
 /* Free and total device memory of HIP device `device`, for admission by footprint. */
 int sampo_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 

@@ -112,6 +112,9 @@ def lib():
         L.sampo_engine_histogram.argtypes = [vp, C.c_uint32, vp]
         L.sampo_engine_agent_steps.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
         L.sampo_engine_time_pass.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.sampo_engine_capture.argtypes = [vp, C.c_uint32]
+        L.sampo_engine_replay.argtypes = [vp]
+        L.sampo_engine_replay_times.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.sampo_stream_copy.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(C.c_float)]
         L.sampo_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.sampo_seed_stream.argtypes = [C.c_uint32, C.c_uint32, vp]
@@ -362,6 +365,20 @@ class Engine:
         ms = C.c_float()
         _check(lib().sampo_engine_time(self.h, n, 1 if graph else 0, C.byref(ms)))
         return ms.value
+
+    def capture(self, n):
+        """Capture the next n iterations as one HIP graph with events around every pass."""
+        _check(lib().sampo_engine_capture(self.h, n))
+
+    def replay(self):
+        """Launch the captured iterations (asynchronous)."""
+        _check(lib().sampo_engine_replay(self.h))
+
+    def replay_times(self):
+        """(first pass start to last pass end ms, summed pass ms) of the replay."""
+        w, p = C.c_float(), C.c_float()
+        _check(lib().sampo_engine_replay_times(self.h, C.byref(w), C.byref(p)))
+        return w.value, p.value
 
     def time_pass(self, n):
         """(wall ms, summed fused-pass ms) over n iterations."""

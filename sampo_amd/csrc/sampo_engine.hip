@@ -1442,6 +1442,10 @@ struct sampo_engine {
 	uint32_t per_cu = 0; /* persistent work-groups per CU; 0: as many as fit */
 	hipGraphExec_t day = nullptr;
 	hipGraph_t day_graph = nullptr;
+	/* sampo_engine_capture: n iterations with an event pair around each pass */
+	hipGraphExec_t cap = nullptr;
+	hipGraph_t cap_graph = nullptr;
+	std::vector<hipEvent_t> cap_ev;
 };
 
 extern "C" const char *sampo_last_error(void) { return g_err; }
@@ -1459,6 +1463,12 @@ static void drop_graph(sampo_engine *e)
 	if (e->day_graph) (void)hipGraphDestroy(e->day_graph);
 	e->day = nullptr;
 	e->day_graph = nullptr;
+	if (e->cap) (void)hipGraphExecDestroy(e->cap);
+	if (e->cap_graph) (void)hipGraphDestroy(e->cap_graph);
+	for (auto &x : e->cap_ev) (void)hipEventDestroy(x);
+	e->cap = nullptr;
+	e->cap_graph = nullptr;
+	e->cap_ev.clear();
 }
 
 static int configure_pass(sampo_engine *e)
@@ -1872,12 +1882,16 @@ extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *g
 
 /* One iteration: the pass over every running instance, then k_prep, which
  * folds it, begins the next iteration and plans the next pass. */
-static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after_pass)
+/* One iteration: the fused pass and k_prep, optionally between two events.
+ * Under stream capture the events are recorded as external event nodes of
+ * the graph, so that they keep their timestamps on replay. */
+static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after_pass, bool capturing = false)
 {
-	if (before_pass) HIPCHK(hipEventRecord(before_pass, e->stream));
+	const unsigned int ef = capturing ? hipEventRecordExternal : hipEventRecordDefault;
+	if (before_pass) HIPCHK(hipEventRecordWithFlags(before_pass, e->stream, ef));
 	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots};
 	HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
-	if (after_pass) HIPCHK(hipEventRecord(after_pass, e->stream));
+	if (after_pass) HIPCHK(hipEventRecordWithFlags(after_pass, e->stream, ef));
 	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
 	                   e->d_pl_par);
 	return 0;
@@ -2160,6 +2174,56 @@ extern "C" int sampo_gen_population(uint64_t seed, uint32_t n0, float t0, sampo_
 }
 
 /* ============================================================ measurement */
+
+extern "C" int sampo_engine_capture(sampo_engine *e, uint32_t n)
+{
+	if (!e || n == 0) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	if (e->cap) (void)hipGraphExecDestroy(e->cap);
+	if (e->cap_graph) (void)hipGraphDestroy(e->cap_graph);
+	for (auto &x : e->cap_ev) (void)hipEventDestroy(x);
+	e->cap = nullptr;
+	e->cap_graph = nullptr;
+	e->cap_ev.assign(2 * (size_t)n, nullptr);
+	for (auto &x : e->cap_ev) HIPCHK(hipEventCreate(&x));
+	HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+	int rc = 0;
+	for (uint32_t i = 0; i < n && !rc; ++i) rc = enqueue_one(e, e->cap_ev[2 * i], e->cap_ev[2 * i + 1], true);
+	hipGraph_t g = nullptr;
+	hipError_t ce = hipStreamEndCapture(e->stream, &g);
+	if (rc) return rc;
+	HIPCHK(ce);
+	e->cap_graph = g;
+	HIPCHK(hipGraphInstantiate(&e->cap, g, nullptr, nullptr, 0));
+	return 0;
+}
+
+extern "C" int sampo_engine_replay(sampo_engine *e)
+{
+	if (!e || !e->cap) return fail(SAMPO_E_STATE, "nothing captured%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipGraphLaunch(e->cap, e->stream));
+	return 0;
+}
+
+extern "C" int sampo_engine_replay_times(sampo_engine *e, float *wall_ms, float *pass_ms)
+{
+	if (!e || !wall_ms || !pass_ms) return fail(SAMPO_E_ARG, "null argument%s%ld");
+	if (!e->cap) return fail(SAMPO_E_STATE, "nothing captured%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	HIPCHK(hipStreamSynchronize(e->stream));
+	const size_t n = e->cap_ev.size() / 2u;
+	float sum = 0.0f;
+	for (size_t i = 0; i < n; ++i) {
+		float ms = 0.0f;
+		HIPCHK(hipEventElapsedTime(&ms, e->cap_ev[2 * i], e->cap_ev[2 * i + 1]));
+		sum += ms;
+	}
+	HIPCHK(hipEventElapsedTime(wall_ms, e->cap_ev[0], e->cap_ev[2 * n - 1]));
+	*pass_ms = sum;
+	return 0;
+}
 
 extern "C" int sampo_engine_time_pass(sampo_engine *e, uint32_t n, float *wall_ms, float *pass_ms)
 {

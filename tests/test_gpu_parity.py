@@ -147,6 +147,33 @@ def test_engine_hip_graph_day_equals_plain_steps():
     agents_equal(out[0][1], out[1][1])
 
 
+def test_engine_captured_iterations_equal_plain_steps():
+    """sampo_engine_capture/replay (one graph of n iterations with events
+    around each pass) gives the plain iterations' results."""
+    steps = 24 * 3 + 5
+    temp, lay, ag = config_inputs(5000, 4, 31)
+    env = S.make_env(cc=5.0 * 5000)
+    dmr = (0.05, 0.03, 0.02, 0.1)
+    out = []
+    for captured in (False, True):
+        with S.Engine(max_slots=1) as e:
+            slot = e.admit(mode=S.MODE_CONFIG, seed=31, horizon=24 * 4, temp=temp, env=env, max_agents=200_000,
+                           dmr=dmr, layout=lay, agents=ag)
+            e.step(7)
+            if captured:
+                e.capture(steps)
+                e.replay()
+                wall, pss = e.replay_times()
+                assert 0.0 < pss <= wall
+            else:
+                e.step(steps)
+            e.sync()
+            out.append((e.rows(slot, 0, 7 + steps), e.export(slot)))
+    rows_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1][0], out[1][1][0])
+    agents_equal(out[0][1][1], out[1][1][1])
+
+
 def test_engine_batched_instances_equal_solo_runs():
     """Instances of different sizes, seeds and modes in one launch per
     iteration are each bit-identical to running alone."""
