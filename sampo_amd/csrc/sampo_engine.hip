@@ -593,15 +593,21 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	/* loads: the run counts of this thread's chunks (when few) and their sums */
 	uint32_t nr[KR][16];
 	uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-	if (in_regs) {
+	if (in_regs) { /* all of this thread's loads in flight at once: one round trip */
 #pragma unroll
 		for (uint32_t j = 0; j < KR; ++j)
 #pragma unroll
 			for (uint32_t r = 0; r < 16u; ++r) nr[j][r] = c_lo + j < c_hi ? count(r, c_lo + j) : 0u;
-	}
-	for (uint32_t c = c_lo; c < c_hi; ++c) {
 #pragma unroll
-		for (int q = 0; q < 8; ++q) v[q] += __builtin_nontemporal_load(co + (size_t)(CO_SUMS + q) * cmax + c);
+		for (uint32_t j = 0; j < KR; ++j)
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				v[q] += c_lo + j < c_hi ? __builtin_nontemporal_load(co + (size_t)(CO_SUMS + q) * cmax + c_lo + j) : 0u;
+	} else {
+		for (uint32_t c = c_lo; c < c_hi; ++c) {
+#pragma unroll
+			for (int q = 0; q < 8; ++q) v[q] += __builtin_nontemporal_load(co + (size_t)(CO_SUMS + q) * cmax + c);
+		}
 	}
 	/* per new state: agents (w) and non-empty runs (ne) of this thread's chunks */
 	uint32_t wv[16];
