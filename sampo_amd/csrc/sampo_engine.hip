@@ -110,8 +110,6 @@ __host__ __device__ static inline bool cut_equal(const Cut &a, const Cut &b)
 #define NOUT 24
 #define CO_SUMS 16
 
-/* destination of a state's movers: the next state, gravid back to BMS */
-__host__ __device__ static inline uint32_t mover_dest(uint32_t s) { return s == S_GRAVID ? (uint32_t)S_BMS : s + 1u; }
 
 struct IterState {
 	uint32_t step, active, cur, n;
@@ -360,18 +358,6 @@ __device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uin
 	a.misc = b[7 * P + p];
 }
 
-__device__ static inline void store_agent(const Ag &a, uint32_t *b, size_t P, uint32_t p)
-{
-	b[p] = sc_bits_f(a.age);
-	b[P + p] = sc_bits_f(a.his);
-	b[2 * P + p] = sc_bits_f(a.dly);
-	b[3 * P + p] = a.aux;
-	b[4 * P + p] = sc_bits_f(a.csd);
-	b[5 * P + p] = a.avail;
-	b[6 * P + p] = a.cyc;
-	b[7 * P + p] = a.misc;
-}
-
 /* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3);
  * inc is eggsIncubation at the laying temperature, thr/m/b the eggsHatching
  * bands */
@@ -395,16 +381,6 @@ __device__ static inline uint32_t wave_sum(uint32_t v)
 #pragma unroll
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
 	return v;
-}
-
-/* last segment j in [lo, hi) with seg k0 <= k (segments sorted by k0) */
-__device__ static inline uint32_t seg_find(const uint32_t *k0s, uint32_t lo, uint32_t hi, uint32_t k)
-{
-	while (hi - lo > 1u) {
-		uint32_t mid = (lo + hi) >> 1;
-		if (k0s[mid] <= k) lo = mid; else hi = mid;
-	}
-	return lo;
 }
 
 /* ------------------------------------------------------------ plan */
