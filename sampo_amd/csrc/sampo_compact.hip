@@ -36,6 +36,7 @@
 
 #include "../../include/sampo_engine.h"
 #include "../../include/sampo_compact.h"
+#include "sampo_wave.h"
 
 #define CT 256u                 /* threads per tile */
 #define CV 16u                  /* keys per thread and round */
@@ -47,12 +48,6 @@
 #define CCNT 2u                 /* tiles per work-group of k_count */
 
 namespace {
-
-__device__ inline uint64_t shfl_up64(uint64_t v, int o)
-{
-	uint32_t lo = __shfl_up((uint32_t)v, o, 64), hi = __shfl_up((uint32_t)(v >> 32), o, 64);
-	return ((uint64_t)hi << 32) | lo;
-}
 
 __device__ inline uint32_t field16(uint64_t w, uint32_t i) { return (uint32_t)(w >> (16u * i)) & 0xffffu; }
 
@@ -137,13 +132,10 @@ __global__ void __launch_bounds__(CT) k_count(const uint8_t *__restrict__ keys, 
 		widen(c8, ta[g], tb[g]);
 	}
 #pragma unroll
-	for (uint32_t g = 0; g < CCNT; ++g) {
-#pragma unroll
-		for (int o = 32; o > 0; o >>= 1) { /* fields stay below 2^16 */
-			ta[g] += ((uint64_t)__shfl_xor((uint32_t)(ta[g] >> 32), o, 64) << 32) | __shfl_xor((uint32_t)ta[g], o, 64);
-			tb[g] += ((uint64_t)__shfl_xor((uint32_t)(tb[g] >> 32), o, 64) << 32) | __shfl_xor((uint32_t)tb[g], o, 64);
-		}
-		if (lane == 0) {
+	for (uint32_t g = 0; g < CCNT; ++g) { /* wave totals; fields stay below 2^16 */
+		ta[g] = wave_incl_scan_fields(ta[g]);
+		tb[g] = wave_incl_scan_fields(tb[g]);
+		if (lane == 63u) {
 			s_w[g][w][0] = ta[g];
 			s_w[g][w][1] = tb[g];
 		}
@@ -165,12 +157,7 @@ __device__ inline void block_scan8(uint32_t (&v)[CK], uint32_t (&total)[CK], uin
 	uint32_t x[CK];
 #pragma unroll
 	for (uint32_t k = 0; k < CK; ++k) {
-		x[k] = v[k];
-#pragma unroll
-		for (int o = 1; o < 64; o <<= 1) {
-			const uint32_t y = __shfl_up(x[k], o, 64);
-			if (lane >= (uint32_t)o) x[k] += y;
-		}
+		x[k] = wave_incl_scan(v[k]);
 		if (lane == 63u) s_w[w][k] = x[k];
 	}
 	__syncthreads();
@@ -266,15 +253,7 @@ __global__ void __launch_bounds__(CT) k_rank(const uint8_t *__restrict__ keys, u
 	for (uint32_t r = 0; r < CR; ++r) {
 		uint64_t ca, cb;
 		count_keys(kw[r], ca, cb);
-		uint64_t xa = ca, xb = cb;
-#pragma unroll
-		for (int o = 1; o < 64; o <<= 1) {
-			const uint64_t ya = shfl_up64(xa, o), yb = shfl_up64(xb, o);
-			if (lane >= (uint32_t)o) {
-				xa += ya;
-				xb += yb;
-			}
-		}
+		const uint64_t xa = wave_incl_scan_fields(ca), xb = wave_incl_scan_fields(cb);
 		if (lane == 63u) {
 			s_w[w][r][0] = xa;
 			s_w[w][r][1] = xb;

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "sampo_model.h"
+#include "sampo_wave.h"
 
 /* ------------------------------------------------------------ device state */
 
@@ -488,22 +489,6 @@ __device__ static inline Cut chunk_cut(const Inst *I, uint32_t N)
 
 /* Exclusive scan of N values per thread over the work-group, all at once;
  * the totals land in tot (shared). */
-/* Inclusive scan over the 64 lanes of a wave with DPP moves (no LDS
- * traffic): Kogge-Stone inside each 16-lane row by row shifts of 1, 2, 4
- * and 8, then row 0's and row 1's last lanes carried into the rows after
- * them by the row broadcasts of lanes 15 and 31.  Lanes whose DPP source is
- * outside the row, and rows masked off, receive 0. */
-__device__ static inline uint32_t wave_incl_scan(uint32_t x)
-{
-	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false); /* row_shr:1 */
-	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false); /* row_shr:2 */
-	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false); /* row_shr:4 */
-	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false); /* row_shr:8 */
-	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); /* row_bcast:15 */
-	x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); /* row_bcast:31 */
-	return x;
-}
-
 template <int N>
 __device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot)
 {
