@@ -133,6 +133,10 @@ struct IterState {
 	 * batch_* are read as arrays through LDS pointers and must stay adjacent */
 	float ms_dly, bms_dly, larva_mean, larva_sd, batch_mean, batch_sd, batch_decay, lay_p, bite_human;
 	float hatch_thr[4], hatch_m[5], hatch_b[5];
+	/* the iteration's 16 seeds (set seed_set) and seeds[2], seeds[3] of the
+	 * set that laid the newborns (nb_set), copied here by the prep so that
+	 * the pass reads them with the rest of this record */
+	uint32_t seed[16], nbseed[8];
 	double csd_inc;        /* 1/((111/(T-16))*14) of updateAgents.cl:69-70, in binary64 */
 	float hmr_larva[100];
 };
@@ -190,10 +194,9 @@ struct Acc {
 struct TileParams {
 	IterState ip;
 	float hmr_adult[100];
-	uint32_t seed[16];
-	uint32_t nbseed[8];   /* seeds[2], seeds[3] of the set that laid the newborns */
 	uint32_t *bin, *bout;
 	const Seg *seg;
+	const Chunk *chk;     /* the instance's chunks of this iteration */
 	uint32_t *cout;
 	Chunk ch;
 	uint32_t P, nseg, cmax;
@@ -232,7 +235,7 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	uint32_t d = sc_f2u(sc_minf(a.age / 24.0f, 99.0f));
 	float hmr = s == S_EGG ? ip.hmr_egg : s == S_LARVA ? ip.hmr_larva[d] : s == S_PUPA ? ip.hmr_pupa
 	                                                                                 : T.hmr_adult[d];
-	if (rnd(&T.seed[0], g) <= hmr) return S_DEAD;
+	if (rnd(&T.ip.seed[0], g) <= hmr) return S_DEAD;
 
 	/* updateAgents.cl:62-70 */
 	const float t = ip.temp;
@@ -249,12 +252,12 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 		if (a.his >= a.dly && (open & (1u << S_EGG))) {
 			a.age = 24.0f;
 			a.his = 0.0f;
-			a.dly = bm_delay(rnd(&T.seed[4], g), rnd(&T.seed[8], g), (lds_fp)&ip.larva_mean);
+			a.dly = bm_delay(rnd(&T.ip.seed[4], g), rnd(&T.ip.seed[8], g), (lds_fp)&ip.larva_mean);
 			return S_LARVA;
 		}
 		return S_EGG;
 	case S_LARVA: { /* :89-113 */
-		if (ip.env.larvacideValue > 0.0f && a.his == 1.0f && rnd(&T.seed[8], g) <= ip.env.larvacideValue)
+		if (ip.env.larvacideValue > 0.0f && a.his == 1.0f && rnd(&T.ip.seed[8], g) <= ip.env.larvacideValue)
 			return S_DEAD;
 		float c = sc_float(a.aux) + ip.dev_rate;
 		a.aux = sc_bits_f(c);
@@ -289,10 +292,10 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 		return S_MS;
 	case S_BMS: /* :159-197 */
 		a.cyc += 1u;
-		if (a.his >= a.dly && (open & (1u << S_BMS)) && rnd(&T.seed[4], g) <= ip.env.bloodmealSuccess) {
+		if (a.his >= a.dly && (open & (1u << S_BMS)) && rnd(&T.ip.seed[4], g) <= ip.env.bloodmealSuccess) {
 			const float itn = ip.itn_kill;
-			if (itn > 0.0f && rnd(&T.seed[8], g) <= itn) return S_DEAD;
-			const uint32_t human = ip.bite_human < 1.0f ? (rnd(&T.seed[12], g) <= ip.bite_human) : 1u;
+			if (itn > 0.0f && rnd(&T.ip.seed[8], g) <= itn) return S_DEAD;
+			const uint32_t human = ip.bite_human < 1.0f ? (rnd(&T.ip.seed[12], g) <= ip.bite_human) : 1u;
 			a.misc = (a.misc & 0xffff0000u) | ((hbc + human) & 0xffffu);
 			acc.bites += 1u;
 			a.his = 0.0f;
@@ -303,12 +306,12 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 		return S_BMS;
 	case S_BMD: { /* :199-228 */
 		const float irs = ip.irs_kill;
-		if (irs > 0.0f && rnd(&T.seed[8], g) <= irs) return S_DEAD;
+		if (irs > 0.0f && rnd(&T.ip.seed[8], g) <= irs) return S_DEAD;
 		a.cyc += 1u;
 		if (a.his >= a.dly && (open & (1u << S_BMD))) {
 			if (a.avail == 0u && (a.misc >> 31)) { /* generateEggs, properties.h:257-266 */
 				uint32_t nb = (a.misc >> 16) & 0x7fffu;
-				a.avail = gen_eggs(rnd(&T.seed[4], g), rnd(&T.seed[8], g), nb, (lds_fp)&ip.batch_mean);
+				a.avail = gen_eggs(rnd(&T.ip.seed[4], g), rnd(&T.ip.seed[8], g), nb, (lds_fp)&ip.batch_mean);
 				a.misc = (a.misc & 0x8000ffffu) | (((nb + 1u) & 0x7fffu) << 16);
 			}
 			a.his = 0.0f;
@@ -319,9 +322,9 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	}
 	default: { /* gravid :230-269 */
 		a.cyc += 1u;
-		if ((open & (1u << S_GRAVID)) && rnd(&T.seed[4], g) < ip.lay_p) {
+		if ((open & (1u << S_GRAVID)) && rnd(&T.ip.seed[4], g) < ip.lay_p) {
 			int dead = 0;
-			if (ip.env.oviTrapValue > 0.0f && rnd(&T.seed[8], g) <= ip.env.oviTrapValue) {
+			if (ip.env.oviTrapValue > 0.0f && rnd(&T.ip.seed[8], g) <= ip.env.oviTrapValue) {
 				dead = 1;
 				a.avail = 0u;
 			}
@@ -777,6 +780,12 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 	}
 	__syncthreads();
 	uint32_t st = s_status;
+	if (st == SAMPO_RUNNING) { /* step < horizon: both seed sets exist */
+		if (tid < 16u)
+			ip.seed[tid] = I->seeds[(size_t)ip.seed_set * 16u + tid];
+		else if (tid < 24u)
+			ip.nbseed[tid - 16u] = I->seeds[(size_t)ip.nb_set * 16u + 8u + (tid - 16u)];
+	}
 	/* chunks (chunk_cut).  Their outputs go to the same dense slot range of
 	 * the other buffer, so N <= P is all the room needed. */
 	const uint32_t N = ip.n;
@@ -936,7 +945,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	constexpr uint32_t TILE = (uint32_t)WG * APT;
 	constexpr uint32_t SEGC = TILE + 64u; /* staged segment window: a tile overlaps <= TILE + 1 */
 	__shared__ TileParams T;
-	__shared__ uint32_t s_ticket, s_inst;
+	__shared__ uint32_t s_ticket, s_inst, s_c;
 	__shared__ uint32_t s_k0[SEGC], s_st[SEGC], s_pb[SEGC], s_vb[SEGC];
 	__shared__ uint32_t s_wlo, s_wm, s_wend; /* window: global index, size, dense end */
 	__shared__ uint32_t s_wc[APT][NW][16];    /* per (input state, moved) */
@@ -1039,50 +1048,76 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 		}
 	};
 
+	/* Tickets: a work-group's first chunk is its own index, later ones come
+	 * from the counter (from gridDim.x on), taken by tid 0 while the
+	 * chunk's last tile is worked on.  T keeps an instance's parameters
+	 * across its chunks: within a launch they do not change. */
+	uint32_t nt = blockIdx.x;
+	uint32_t have = 0xffffffffu;
 	for (;;) {
-		if (tid == 0) {
-			uint32_t t = atomicAdd(&ctx->ticket, 1u);
-			s_ticket = t;
-			if (t < total) { /* last instance whose base <= t */
-				uint32_t lo = 0, hi = ninst;
+		if (w == 0) { /* the instance holding ticket t: the last one whose base <= t, 64 probes a round */
+			const uint32_t t = __builtin_amdgcn_readfirstlane(nt);
+			uint32_t lo = 0, base = 0;
+			if (t < total) {
+				uint32_t hi = ninst;
 				while (hi - lo > 1u) {
-					uint32_t mid = (lo + hi) >> 1;
-					if (pl_base[mid] <= t) lo = mid; else hi = mid;
+					const uint32_t step = (hi - lo + 63u) / 64u;
+					const uint32_t j = lo + lane * step;
+					const uint32_t b = j < hi ? pl_base[j] : 0xffffffffu; /* lane 0 probes lo: base <= t */
+					const uint64_t m = __ballot(b <= t);
+					const uint32_t J = 63u - (uint32_t)__clzll((unsigned long long)m);
+					base = __shfl(b, (int)J, 64);
+					const uint32_t nlo = lo + J * step;
+					hi = nlo + step < hi ? nlo + step : hi;
+					lo = nlo;
 				}
+			}
+			if (lane == 0) {
+				s_ticket = t;
 				s_inst = lo;
+				s_c = t - base;
 			}
 		}
-		__syncthreads();
+		lds_barrier();
 		const uint32_t ticket = s_ticket;
 		if (ticket >= total) break;
-		const uint32_t ii = s_inst;
-		const uint32_t c = ticket - pl_base[ii];
-		{ /* parameters into LDS */
-			Inst *I = insts + ii;
+		const uint32_t ii = s_inst, c = s_c;
+		if (ii != have) { /* the instance's parameters into LDS; both parities are
+		                     loaded at once, with the parity, and one is kept */
+			have = ii;
+			const Inst *I = insts + ii;
 			const uint32_t par = pl_par[ii];
-			const uint32_t *src = (const uint32_t *)&I->st[par];
+			const uint32_t *s0 = (const uint32_t *)&I->st[0], *s1 = (const uint32_t *)&I->st[1];
 			uint32_t *dst = (uint32_t *)&T.ip;
-			for (uint32_t q = tid; q < sizeof(IterState) / 4u; q += WG) dst[q] = src[q];
+			for (uint32_t q = tid; q < sizeof(IterState) / 4u; q += WG) {
+				uint32_t x0 = s0[q], x1 = s1[q];
+				/* opaque values: the select must not become a load at a parity-dependent address */
+				__asm__("" : "+v"(x0), "+v"(x1));
+				dst[q] = par ? x1 : x0;
+			}
 			for (uint32_t q = tid; q < 100u; q += WG) T.hmr_adult[q] = I->hmr_adult[q];
-			const uint32_t *csrc = (const uint32_t *)&I->chunk[par][c];
-			uint32_t *cdst = (uint32_t *)&T.ch;
-			for (uint32_t q = tid; q < sizeof(Chunk) / 4u; q += WG) cdst[q] = csrc[q];
-			if (tid < 16u) T.seed[tid] = I->seeds[(size_t)(I->st[par].seed_set) * 16u + tid];
-			if (tid >= 32u && tid < 40u)
-				T.nbseed[tid - 32u] = I->seeds[(size_t)(I->st[par].nb_set) * 16u + 8u + (tid - 32u)];
 			if (tid == 0) {
-				uint32_t cur = I->st[par].cur;
-				T.bin = I->buf[cur];
-				T.bout = I->buf[cur ^ 1u];
-				T.seg = I->seg[par];
+				uint32_t *b0 = I->buf[0], *b1 = I->buf[1];
+				const Seg *g0 = I->seg[0], *g1 = I->seg[1];
+				const Chunk *c0 = I->chunk[0], *c1 = I->chunk[1];
+				uint32_t cu0 = I->st[0].cur, cu1 = I->st[1].cur, ns0 = I->st[0].nseg, ns1 = I->st[1].nseg;
+				__asm__("" : "+v"(b0), "+v"(b1), "+v"(g0), "+v"(g1), "+v"(c0), "+v"(c1));
+				__asm__("" : "+v"(cu0), "+v"(cu1), "+v"(ns0), "+v"(ns1));
+				const uint32_t cur = par ? cu1 : cu0;
+				T.bin = cur ? b1 : b0;
+				T.bout = cur ? b0 : b1;
+				T.seg = par ? g1 : g0;
+				T.chk = par ? c1 : c0;
+				T.nseg = par ? ns1 : ns0;
 				T.cout = I->cout;
 				T.cmax = I->cmax;
 				T.P = I->P;
-				T.nseg = I->st[par].nseg;
 			}
-			if (tid < 16u) s_run[tid] = 0u;
+			lds_barrier();
 		}
-		__syncthreads();
+		if (tid < 4u) ((uint32_t *)&T.ch)[tid] = ((const gu32 *)(T.chk + c))[tid];
+		if (tid < 16u) s_run[tid] = 0u;
+		lds_barrier();
 		if (tid < 8u) {
 			uint32_t h = T.ip.kstart[tid] > T.ch.k0 ? T.ip.kstart[tid] : T.ch.k0;
 			uint32_t z = T.ip.kstart[tid + 1u] < T.ch.k1 ? T.ip.kstart[tid + 1u] : T.ch.k1;
@@ -1131,7 +1166,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				me[jx].g = s_sg[o];
 				if (me[jx].s < S_DEAD) {
 					if (me[jx].born) {
-						newborn(a[jx], T.nbseed, T.ip.nb_inc, T.ip.hatch_thr, T.ip.hatch_m, T.ip.hatch_b, me[jx].g);
+						newborn(a[jx], T.ip.nbseed, T.ip.nb_inc, T.ip.hatch_thr, T.ip.hatch_m, T.ip.hatch_b, me[jx].g);
 					} else {
 						a[jx].age = sc_float(cur[F_AGE][o]);
 						a[jx].his = sc_float(cur[F_HIS][o]);
@@ -1156,6 +1191,8 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				if (w == 0 && te2 > s_wend) stage(next, te2);
 				lds_barrier();
 				issue(ts2, te2, nxt, fin);
+			} else if (tid == 0) {
+				nt = gridDim.x + atomicAdd(&ctx->ticket, 1u);
 			}
 			uint32_t ns[APT], lr[APT];
 #pragma unroll 1
@@ -1233,7 +1270,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				uint32_t r = wave_sum(v[q]);
 				if (lane == 0) s_acc[w][q] = r;
 			}
-			__syncthreads();
+			lds_barrier();
 			if (tid < 8u) {
 				uint32_t r = 0;
 				for (int q = 0; q < NW; ++q) r += s_acc[q][tid];
@@ -1243,7 +1280,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				((gu32 *)T.cout)[(size_t)((tid & 1u) * 8u + (tid >> 1)) * T.cmax + c] = s_run[tid];
 			if (tid == 0) insts[ii].pending = 1u;
 		}
-		__syncthreads();
+		lds_barrier();
 	}
 }
 
