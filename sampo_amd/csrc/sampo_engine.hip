@@ -45,6 +45,66 @@
 
 /* ------------------------------------------------------------ device state */
 
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void glob_void;
+/* Global-memory views of pointers read from LDS: accesses through them are
+ * global (not flat) instructions, which the compiler knows cannot touch LDS
+ * -- so they never make it wait for global memory before an LDS access, and
+ * loads of one round trip are issued together. */
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4 gu32x4;
+template <class V> __device__ static inline __attribute__((address_space(1))) V *gmem(V *p)
+{
+	return (__attribute__((address_space(1))) V *)p;
+}
+/* Barrier for LDS traffic only: it waits for the wave's LDS instructions
+ * (lgkmcnt) alone, not for its global loads and stores (a __syncthreads
+ * waits for every store to complete).  For work-groups whose waves never
+ * exchange data through global memory: the pass, whose direct-to-LDS loads
+ * land in slots only the issuing lane reads after its own wait, so the next
+ * tile's loads stay in flight across the current tile's barriers; and the
+ * prep, whose global reads never follow a global write of the same launch
+ * (a single instance is planned from its LDS record).  The empty asm
+ * statements keep the compiler from moving memory accesses across it. */
+__device__ static inline void lds_barrier()
+{
+	__asm__ __volatile__("" ::: "memory");
+	__builtin_amdgcn_s_waitcnt(0xC07F); /* lgkmcnt(0) */
+	__builtin_amdgcn_s_barrier();
+	__asm__ __volatile__("" ::: "memory");
+}
+
+/* A pointer every lane holds the same value of, moved to scalar registers,
+ * so that addressing is scalar base + 32-bit lane offset. */
+__device__ static inline gu32 *uniform_ptr(const uint32_t *p)
+{
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+	return (gu32 *)(((uint64_t)hi << 32) | lo);
+}
+
+/* whole records through the global view, word by word */
+template <class V> __device__ static inline void gput(V *dst, const V &v)
+{
+	static_assert(sizeof(V) % 4u == 0u, "word-sized records");
+	gu32 *d = (gu32 *)dst;
+	const uint32_t *s = (const uint32_t *)&v;
+#pragma unroll
+	for (unsigned i = 0; i < sizeof(V) / 4u; ++i) d[i] = s[i];
+}
+template <class V> __device__ static inline V gget(const V *src)
+{
+	static_assert(sizeof(V) % 4u == 0u, "word-sized records");
+	V v;
+	const gu32 *s = (const gu32 *)src;
+	uint32_t *d = (uint32_t *)&v;
+#pragma unroll
+	for (unsigned i = 0; i < sizeof(V) / 4u; ++i) d[i] = s[i];
+	return v;
+}
+
 #define PB_NEWBORN 0xffffffffu
 #define SEG_REV 0x100u /* in Seg.state: the run is stored backwards from pbase */
 
@@ -467,7 +527,7 @@ __global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *p
  * of them.  When that makes chunks of 4 x chunk_size or more, the first
  * target/2 chunks take 80 % of the agents and the rest are chunks of a
  * quarter of their size. */
-__device__ static inline Cut chunk_cut(const Inst *I, uint32_t N)
+__device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_t N)
 {
 	const uint32_t cs = I->chunk_size, T = I->target_chunks;
 	Cut k = {cs, cs, 0u};
@@ -491,32 +551,39 @@ __device__ static inline Cut chunk_cut(const Inst *I, uint32_t N)
 }
 
 /* Exclusive scan of N values per thread over the work-group, all at once;
- * the totals land in tot (shared). */
+ * the totals land in tot (shared).  Waves scan with DPP; the wave totals of
+ * column i are then scanned by row i % 4 of wave 0, 16 lanes for the 16
+ * waves, four columns a round. */
 template <int N>
 __device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot)
 {
-	__shared__ uint32_t lds[PREP_WG / 64][N];
+	constexpr uint32_t NW = PREP_WG / 64u;
+	static_assert(NW <= 16u, "one 16-lane row holds a column's wave totals");
+	__shared__ uint32_t lds[NW][N];
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-	uint32_t x[N];
 #pragma unroll
-	for (int i = 0; i < N; ++i) x[i] = wave_incl_scan(v[i]);
-	if (lane == 63u)
-#pragma unroll
-		for (int i = 0; i < N; ++i) lds[w][i] = x[i];
-	__syncthreads();
-	if (tid < (uint32_t)N) { /* wave totals -> exclusive wave offsets, per column */
-		uint32_t run = 0;
-		for (uint32_t q = 0; q < PREP_WG / 64; ++q) {
-			const uint32_t t = lds[q][tid];
-			lds[q][tid] = run;
-			run += t;
-		}
-		tot[tid] = run;
+	for (int i = 0; i < N; ++i) { /* exclusive within the wave; lane 63 posts the wave total */
+		const uint32_t x = wave_incl_scan(v[i]);
+		if (lane == 63u) lds[w][i] = x;
+		ex[i] = x - v[i];
 	}
-	__syncthreads();
+	lds_barrier();
+	if (w == 0) {
+		const uint32_t q = lane & 15u, r = lane >> 4;
 #pragma unroll
-	for (int i = 0; i < N; ++i) ex[i] = lds[w][i] + x[i] - v[i];
-	__syncthreads();
+		for (uint32_t i0 = 0; i0 < (uint32_t)N; i0 += 4u) {
+			const uint32_t i = i0 + r;
+			const bool in = i < (uint32_t)N && q < NW;
+			const uint32_t y = in ? lds[q][i] : 0u;
+			const uint32_t s = row_incl_scan(y);
+			if (in) lds[q][i] = s - y;
+			if (q == 15u && i < (uint32_t)N) tot[i] = s;
+		}
+	}
+	lds_barrier();
+#pragma unroll
+	for (int i = 0; i < N; ++i) ex[i] += lds[w][i];
+	lds_barrier();
 }
 
 /* The runs of one chunk in the next iteration's dense order: for each new
@@ -540,43 +607,51 @@ __device__ static inline uint32_t run_row(uint32_t sn, uint32_t kind)
 __device__ static void fold_iteration(Inst *I, uint32_t par)
 {
 	constexpr uint32_t KR = 2; /* chunks per thread held in registers */
-	__shared__ uint32_t s_sum[8], s_kst[9], s_vst[8], s_sb[8], s_tot[16], s_C;
+	__shared__ uint32_t s_kst[9], s_vst[8], s_sb[8], s_tot[24], s_C;
 	__shared__ Cut s_cut;
-	const uint32_t tid = threadIdx.x, lane = tid & 63u;
+	const uint32_t tid = threadIdx.x;
 	const IterState &ip = I->st[par];
 	IterState &np = I->st[par ^ 1u];
 	const uint32_t C = ip.nchunk, cmax = I->cmax;
-	const uint32_t *co = I->cout;
-	Chunk *nch = I->chunk[par ^ 1u];
-	Seg *seg = I->seg[par ^ 1u];
-	if (tid < 8u) s_sum[tid] = 0u;
+	const gu32 *co = uniform_ptr(I->cout); /* NOUT * cmax words: 32-bit indices */
+	auto *nch = gmem(I->chunk[par ^ 1u]);
+	Seg *seg = I->seg[par ^ 1u]; /* written with gput */
 	const uint32_t K = (C + PREP_WG - 1u) / PREP_WG;
 	const uint32_t c_lo = tid * K < C ? tid * K : C, c_hi = c_lo + K < C ? c_lo + K : C;
 	const bool in_regs = K <= KR;
-	auto count = [&](uint32_t row, uint32_t c) -> uint32_t { return co[(size_t)row * cmax + c]; };
-	/* loads: the run counts of this thread's chunks (when few) and their sums */
-	uint32_t nr[KR][16];
+	auto count = [&](uint32_t row, uint32_t c) -> uint32_t { return co[row * cmax + c]; };
+	/* loads: the run counts of this thread's chunks (when few) and their sums.
+	 * One 16-entry array per chunk: read by a uniform index, each stays in
+	 * registers. */
+	static_assert(KR == 2, "two register-held chunks per thread");
+	uint32_t nr0[16], nr1[16];
 	uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-	if (in_regs) { /* all of this thread's loads in flight at once: one round trip */
+	if (in_regs) { /* all of this thread's loads in flight at once: one round trip.  They are
+	                  unconditional (column 0 stands in past c_hi) and masked afterwards */
 #pragma unroll
-		for (uint32_t j = 0; j < KR; ++j)
+		for (uint32_t j = 0; j < KR; ++j) {
+			const uint32_t cj = c_lo + j < c_hi ? c_lo + j : 0u;
 #pragma unroll
-			for (uint32_t r = 0; r < 16u; ++r) nr[j][r] = c_lo + j < c_hi ? count(r, c_lo + j) : 0u;
+			for (uint32_t r = 0; r < 16u; ++r) (j ? nr1 : nr0)[r] = count(r, cj);
 #pragma unroll
-		for (uint32_t j = 0; j < KR; ++j)
+			for (uint32_t q = 0; q < 8u; ++q) v[q] += c_lo + j < c_hi ? co[(CO_SUMS + q) * cmax + cj] : 0u;
+		}
 #pragma unroll
-			for (int q = 0; q < 8; ++q)
-				v[q] += c_lo + j < c_hi ? __builtin_nontemporal_load(co + (size_t)(CO_SUMS + q) * cmax + c_lo + j) : 0u;
+		for (uint32_t r = 0; r < 16u; ++r) {
+			nr0[r] = c_lo < c_hi ? nr0[r] : 0u;
+			nr1[r] = c_lo + 1u < c_hi ? nr1[r] : 0u;
+		}
 	} else {
 		for (uint32_t c = c_lo; c < c_hi; ++c) {
 #pragma unroll
-			for (int q = 0; q < 8; ++q) v[q] += __builtin_nontemporal_load(co + (size_t)(CO_SUMS + q) * cmax + c);
+			for (uint32_t q = 0; q < 8u; ++q) v[q] += co[(CO_SUMS + q) * cmax + c];
 		}
 	}
-	/* per new state: agents (w) and non-empty runs (ne) of this thread's chunks */
-	uint32_t wv[16];
+	/* per new state: agents (wv[sn]) and non-empty runs (wv[8 + sn]) of this
+	 * thread's chunks; then the 8 sums, scanned along for their totals */
+	uint32_t wv[24];
 #pragma unroll
-	for (int i = 0; i < 16; ++i) wv[i] = 0u;
+	for (int i = 0; i < 24; ++i) wv[i] = i < 16 ? 0u : v[i - 16];
 	if (in_regs) {
 #pragma unroll
 		for (uint32_t j = 0; j < KR; ++j)
@@ -585,7 +660,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 #pragma unroll
 				for (uint32_t kind = 0; kind < 3u; ++kind) {
 					if (kind >= runs_of(sn)) continue;
-					const uint32_t n = nr[j][run_row(sn, kind)]; /* zero past c_hi */
+					const uint32_t n = (j ? nr1 : nr0)[run_row(sn, kind)]; /* zero past c_hi */
 					wv[sn] += n;
 					wv[8u + sn] += n ? 1u : 0u;
 				}
@@ -601,13 +676,9 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 					wv[8u + sn] += n ? 1u : 0u;
 				}
 	}
-#pragma unroll
-	for (int q = 0; q < 8; ++q) {
-		uint32_t r = wave_sum(v[q]);
-		if (lane == 0 && r) atomicAdd(&s_sum[q], r);
-	}
-	uint32_t ex[16];
-	block_excl_n<16>(wv, ex, s_tot);
+	uint32_t ex[24];
+	block_excl_n<24>(wv, ex, s_tot);
+	const uint32_t *s_sum = s_tot + 16; /* newEggs, bites, ..., potentially infective */
 	const uint32_t E = s_sum[0];
 	if (tid == 0) {
 		uint32_t k = 0, vend = 0, sb = E > 0u ? 1u : 0u;
@@ -642,7 +713,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		np.cut = nk;
 		np.seg0_done = 1u;
 	}
-	__syncthreads();
+	lds_barrier();
 	const Cut cut = s_cut, oc = ip.cut; /* chunks of the next and of the finished iteration */
 	const uint32_t NC = s_C, on = ip.n;
 	/* segments: the newborns, then per new state the non-empty runs of
@@ -650,13 +721,14 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	 * start is passed. */
 	if (E > 0u) {
 		if (tid == 0) {
-			Seg sg = {0u, 0u, PB_NEWBORN, 0u};
-			seg[0] = sg;
+			const Seg sg = {0u, 0u, PB_NEWBORN, 0u};
+			gput(&seg[0], sg);
 		}
 		for (uint32_t c = tid; c < NC && cut_start(cut, c) < E; c += PREP_WG) nch[c].seg0 = 0u;
 	}
 	if (c_lo >= c_hi) return;
 	/* one run: its segment at dense k0, index idx, and the chunk starts it covers */
+	gu32x4 *segv = (gu32x4 *)seg;
 	auto emit = [&](uint32_t sn, uint32_t row, uint32_t c, uint32_t n, uint32_t &k0, uint32_t &idx, uint32_t &nxc,
 	                uint32_t &nxk) {
 		const uint32_t src = row & 7u, mv = row >> 3;
@@ -664,12 +736,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		const uint32_t ck0 = cut_start(oc, c), ce = cut_start(oc, c + 1u), ck1 = ce < on ? ce : on;
 		const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
 		const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
-		Seg sg;
-		sg.k0 = k0;
-		sg.state = sn | (mv ? SEG_REV : 0u);
-		sg.pbase = mv ? be - 1u : bh;
-		sg.vbase = s_vst[sn] - s_kst[sn] + k0;
-		seg[idx] = sg;
+		segv[idx] = u32x4{k0, sn | (mv ? SEG_REV : 0u), mv ? be - 1u : bh, s_vst[sn] - s_kst[sn] + k0}; /* Seg */
 		for (; nxc < NC && nxk < k0 + n; ++nxc, nxk = cut_start(cut, nxc)) nch[nxc].seg0 = idx;
 		++idx;
 		k0 += n;
@@ -687,7 +754,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 				for (uint32_t kind = 0; kind < 3u; ++kind) {
 					if (kind >= runs_of(sn)) continue;
 					const uint32_t row = run_row(sn, kind);
-					const uint32_t n = nr[j][row];
+					const uint32_t n = (j ? nr1 : nr0)[row];
 					if (n) emit(sn, row, c_lo + j, n, k0, idx, nxc, nxk);
 				}
 			}
@@ -714,6 +781,16 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 	__shared__ uint32_t s_status, s_go;
 	const uint32_t tid = threadIdx.x;
 	IterState &ip = I->st[par];
+	if (tid >= 64u && tid < 88u && ip.step < I->horizon) {
+		/* the seed sets step + 1 (seed_set) and step (nb_set) of a running
+		 * iteration (step < horizon: both exist), loaded beside tid 0's work */
+		const uint32_t q = tid - 64u;
+		const gu32 *z = gmem(I->seeds);
+		if (q < 16u)
+			ip.seed[q] = z[(size_t)(ip.step + 1u) * 16u + q];
+		else
+			ip.nbseed[q - 16u] = z[(size_t)ip.step * 16u + 8u + (q - 16u)];
+	}
 	if (tid == 0) {
 		uint32_t n = 0;
 		for (int s = 0; s < 8; ++s) {
@@ -747,17 +824,17 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		} else {
 			st = SAMPO_RUNNING;
 		}
-		I->rows[ip.step] = row;
+		gput(&I->rows[ip.step], row);
 		const uint32_t t = ip.step < I->horizon ? ip.step : I->horizon - 1u;
 		const uint32_t tp = ip.step == 0u ? 0u : (ip.step - 1u < I->horizon ? ip.step - 1u : I->horizon - 1u);
-		ip.temp = I->temp[t];
-		const SpRow sr = I->sprow[t];
-		ip.nb_inc = I->sprow[tp].inc;
+		ip.temp = gmem(I->temp)[t];
+		const SpRow sr = gget(&I->sprow[t]);
+		ip.nb_inc = gmem(I->sprow)[tp].inc;
 		{
 			float q = 111.0f / (ip.temp - 16.0f);
 			ip.csd_inc = 1.0 / ((double)q * 14.0);
 		}
-		ip.env = I->env[I->env_count == 1u ? 0u : t];
+		ip.env = gget(&I->env[I->env_count == 1u ? 0u : t]);
 		ip.open = sr.open;
 		ip.dev_rate = sr.dev_rate;
 		ip.pupa_dly = sr.pupa_dly;
@@ -778,29 +855,23 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		}
 		s_status = st;
 	}
-	__syncthreads();
+	lds_barrier();
 	uint32_t st = s_status;
-	if (st == SAMPO_RUNNING) { /* step < horizon: both seed sets exist */
-		if (tid < 16u)
-			ip.seed[tid] = I->seeds[(size_t)ip.seed_set * 16u + tid];
-		else if (tid < 24u)
-			ip.nbseed[tid - 16u] = I->seeds[(size_t)ip.nb_set * 16u + 8u + (tid - 16u)];
-	}
 	/* chunks (chunk_cut).  Their outputs go to the same dense slot range of
 	 * the other buffer, so N <= P is all the room needed. */
 	const uint32_t N = ip.n;
 	const Cut cut = chunk_cut(I, N);
 	const uint32_t C = st == SAMPO_RUNNING ? cut_at_or_after(cut, N) : 0u;
 	const bool have_seg0 = ip.seg0_done && cut_equal(ip.cut, cut);
-	Chunk *ch = I->chunk[par];
-	const Seg *seg = I->seg[par];
+	auto *ch = gmem(I->chunk[par]);
+	const auto *seg = gmem(I->seg[par]);
 	const uint32_t nseg = ip.nseg;
 	for (uint32_t c = tid; c < C; c += PREP_WG) {
 		const uint32_t e = cut_start(cut, c + 1u);
 		ch[c].k0 = cut_start(cut, c);
 		ch[c].k1 = e < N ? e : N;
 	}
-	__syncthreads();
+	lds_barrier();
 	/* initial populations: segment holding each chunk's first agent, segment
 	 * j holding the chunks whose start falls in [k0_j, k0_(j+1)) */
 	if (!have_seg0)
@@ -809,7 +880,7 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 			uint32_t z = j + 1u < nseg ? seg[j + 1u].k0 : N;
 			for (uint32_t c = cut_at_or_after(cut, a); c < C && cut_start(cut, c) < z; ++c) ch[c].seg0 = j;
 		}
-	__syncthreads();
+	lds_barrier();
 	if (tid == 0) {
 		ip.cut = cut;
 		ip.active = st == SAMPO_RUNNING;
@@ -818,7 +889,7 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		I->step = ip.step;
 		s_go = st == SAMPO_RUNNING ? 1u + (I->mode == SAMPO_MODE_REFERENCE ? 1u : 0u) : 0u;
 	}
-	__syncthreads();
+	lds_barrier();
 	if (s_go)
 		for (uint32_t d = tid; d < 100u; d += PREP_WG)
 			ip.hmr_larva[d] = s_go == 2u ? sm_hmr_larva(d, I->l1de, ip.env.carryingCapacity) : I->hmr_stage[1];
@@ -836,7 +907,7 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 		uint32_t *dst = (uint32_t *)&sI;
 		for (uint32_t q = threadIdx.x; q < sizeof(Inst) / 4u; q += PREP_WG) dst[q] = src[q];
 	}
-	__syncthreads();
+	lds_barrier();
 	Inst *I = &sI;
 	if (threadIdx.x == 0) {
 		s_fold = I->status == SAMPO_RUNNING && I->pending;
@@ -845,11 +916,11 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 		/* the iteration being folded was run over this many agents */
 		if (s_fold) atomicAdd(&ctx->agent_steps, (unsigned long long)I->st[I->step & 1u].n);
 	}
-	__syncthreads();
+	lds_barrier();
 	if (s_fold || s_begin) {
 		if (s_fold) {
 			fold_iteration(I, s_par);
-			__syncthreads();
+			lds_barrier();
 			if (threadIdx.x == 0) {
 				I->pending = 0u;
 				I->step = I->step + 1u;
@@ -857,29 +928,38 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 				s_par ^= 1u;
 			}
 		}
-		__syncthreads();
+		lds_barrier();
 		if (s_begin) {
 			begin_iteration(I, s_par);
 			if (threadIdx.x == 0) I->needs_begin = 0u;
 		}
-		__syncthreads();
+		lds_barrier();
 		const uint32_t *src = (const uint32_t *)&sI;
 		uint32_t *dst = (uint32_t *)G;
 		for (uint32_t q = threadIdx.x; q < sizeof(Inst) / 4u; q += PREP_WG) dst[q] = src[q];
 	}
-	/* the last work-group to finish plans the next pass */
+	if (gridDim.x == 1u) { /* alone: the next pass is planned from the LDS record */
+		if (threadIdx.x == 0) {
+			const uint32_t par = sI.step & 1u;
+			const uint32_t nc = sI.status == SAMPO_RUNNING && sI.st[par].active ? sI.st[par].nchunk : 0u;
+			pl_par[0] = (uint8_t)par;
+			pl_base[0] = 0u;
+			pl_base[1] = nc;
+			ctx->total = nc;
+			ctx->ticket = 0u;
+		}
+		return;
+	}
+	/* the last work-group to finish plans the next pass; every store of the
+	 * copy-out is complete (__syncthreads) and published (fence) first */
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		if (gridDim.x == 1u) {
-			s_last = 1u; /* alone: nothing to publish to other work-groups */
-		} else {
-			__threadfence();
-			s_last = atomicAdd(&ctx->prep_done, 1u) == gridDim.x - 1u;
-		}
+		__threadfence();
+		s_last = atomicAdd(&ctx->prep_done, 1u) == gridDim.x - 1u;
 	}
 	__syncthreads();
 	if (s_last) {
-		if (gridDim.x > 1u) __threadfence();
+		__threadfence();
 		plan_all(insts, ctx, pl_base, pl_par, gridDim.x);
 		if (threadIdx.x == 0) ctx->prep_done = 0u;
 	}
@@ -887,42 +967,11 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 
 /* ------------------------------------------------------------ k_pass */
 
-typedef __attribute__((address_space(3))) void lds_void;
-typedef __attribute__((address_space(1))) void glob_void;
-/* Global-memory views of pointers the pass reads from LDS: accesses through
- * them are global (not flat) instructions, which the compiler knows cannot
- * touch LDS -- so they never make it wait before an LDS access. */
-typedef __attribute__((address_space(1))) uint32_t gu32;
-
-/* A pointer every lane holds the same value of, moved to scalar registers,
- * so that addressing is scalar base + 32-bit lane offset. */
-__device__ static inline gu32 *uniform_ptr(const uint32_t *p)
-{
-	const uint64_t v = (uint64_t)p;
-	const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-	const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-	return (gu32 *)(((uint64_t)hi << 32) | lo);
-}
 
 /* &base[slot] for slot < 2^30 (P is capped there): a 32-bit byte offset */
 __device__ static inline gu32 *at_slot(gu32 *base, uint32_t slot)
 {
 	return (gu32 *)((__attribute__((address_space(1))) char *)base + (uint32_t)(slot << 2));
-}
-
-/* Barrier for LDS traffic only.  The pass never exchanges global memory
- * between the waves of a work-group, and the direct-to-LDS loads of a tile
- * land in slots only the issuing lane reads (after its own wait on them), so
- * a barrier waits for the wave's LDS instructions (lgkmcnt) alone -- which
- * lets the next tile's loads stay in flight across the current tile's
- * barriers.  The empty asm statements keep the compiler from moving memory
- * accesses across the barrier. */
-__device__ static inline void lds_barrier()
-{
-	__asm__ __volatile__("" ::: "memory");
-	__builtin_amdgcn_s_waitcnt(0xC07F); /* lgkmcnt(0) */
-	__builtin_amdgcn_s_barrier();
-	__asm__ __volatile__("" ::: "memory");
 }
 
 /* One agent of a tile: where it comes from. */
