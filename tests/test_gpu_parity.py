@@ -200,6 +200,43 @@ def test_engine_batched_instances_equal_solo_runs():
             agents_equal(a, e.export(s)[1])
 
 
+def test_engine_many_instances_match_solo_runs():
+    """150 instances in one launch (more than one wave of 64 in the pass's
+    instance search and in the plan's scan), with horizons of 1 to 4 days so
+    that finished instances hold no chunks between running ones: each is the
+    same whether admitted first or last, and a sample equals its run alone."""
+    n, days = 150, 4
+    specs = []
+    for i in range(n):
+        d = 1 + i % days
+        temp, lay, ag = config_inputs(200 + 37 * (i % 80), d, 300 + i)
+        specs.append(dict(mode=S.MODE_CONFIG, seed=300 + i, horizon=24 * d, temp=temp,
+                          env=S.make_env(cc=5.0 * (200 + 37 * (i % 80))), max_agents=40_000,
+                          dmr=(0.05, 0.03, 0.02, 0.1), layout=lay, agents=ag))
+    res = []
+    for order in (range(n), range(n - 1, -1, -1)):
+        with S.Engine(max_slots=n) as e:
+            e.set_chunk(256, 1 << 20)
+            slots = {i: e.admit(**specs[i]) for i in order}
+            e.step_days(days)
+            got = {}
+            for i, s in slots.items():
+                st, step, _ = e.status(s)
+                assert st == S.DONE and step == specs[i]["horizon"], (i, st, step)
+                got[i] = (e.rows(s, 0, specs[i]["horizon"] + 1), e.export(s)[1])
+            res.append(got)
+    for i in range(n):
+        rows_equal(res[0][i][0], res[1][i][0])
+        agents_equal(res[0][i][1], res[1][i][1])
+    for i in range(0, n, 15):
+        with S.Engine(max_slots=1) as e:
+            e.set_chunk(256, 1 << 20)
+            s = e.admit(**specs[i])
+            e.step(specs[i]["horizon"])
+            rows_equal(e.rows(s, 0, specs[i]["horizon"] + 1), res[0][i][0])
+            agents_equal(e.export(s)[1], res[0][i][1])
+
+
 @pytest.mark.parametrize("chunk", [64, 200, 1000])
 def test_engine_small_chunks_match_oracle(chunk):
     """Many chunks per state: blocks split across chunks, tiles that span
