@@ -175,7 +175,7 @@ def serve_main(args):
     offered = sum(sp.n0 * sp.horizon for sp in specs)
     log(f"serve: {len(specs)} requests over {args.window:.0f} s at {args.rate:.0f}/s, "
         f"{offered / 1e12:.2f}e12 initial-size agent-steps offered")
-    srv = V.Server(max_slots=args.slots, log=log)
+    srv = V.Server(max_slots=args.slots, log=log, policy=args.policy, min_active=args.min_active)
     results, summ = srv.run(specs, clock="wall")
     out = {
         "metric": METRIC, "value": summ["agent_steps_per_s"], "unit": "agent-steps/s", "n_gpus": 1,
@@ -184,7 +184,8 @@ def serve_main(args):
         "config": {"workload": "serving: Poisson arrivals, initial population log-uniform 1e4-1e7, horizon "
                                "30-365 days, temperature/mortality as the sweep, CC = 0.0125 x N0",
                    "rate_per_s": args.rate, "arrival_window_s": args.window, "requests": len(specs),
-                   "slots": args.slots, "parallelism": "1 GPU"},
+                   "slots": args.slots, "policy": args.policy, "min_active_agents": args.min_active,
+                   "parallelism": "1 GPU"},
         "latency_s": {"p50": summ.get("p50"), "p99": summ.get("p99"), "mean": summ.get("mean"),
                       "max": summ.get("max")},
         "serving": summ,
@@ -208,6 +209,10 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="serve: arrivals per second")
     ap.add_argument("--window", type=float, default=10.0, help="serve: seconds of arrivals")
     ap.add_argument("--slots", type=int, default=2048, help="serve: engine slots")
+    ap.add_argument("--policy", default="srpt", choices=["srpt", "all"],
+                    help="serve: advance the requests with the least work left first, or all every day")
+    ap.add_argument("--min-active", type=int, default=64_000_000,
+                    help="serve, srpt: agents kept in each day's step at least")
     ap.add_argument("--tile", default="512x1x6", help="work-group size x agents per thread x min waves per SIMD")
     ap.add_argument("--per-cu", type=int, default=0, help="persistent work-groups of the pass per CU (0: all that fit)")
     ap.add_argument("--chunk", type=int, default=4096, help="single run: minimum agents per chunk")

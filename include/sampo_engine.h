@@ -38,6 +38,7 @@ extern "C" {
 #define SAMPO_EXTINCT 3u       /* nAgents == 0 (src/abms.c:307) */
 #define SAMPO_PHYS_FULL 4u     /* physical slots of the instance exhausted */
 #define SAMPO_GROW 5u          /* paused: its buffers need more slots; sampo_engine_service() grows them */
+#define SAMPO_HELD 6u          /* paused by sampo_engine_hold() until it is resumed */
 #define SAMPO_EMPTY 0xffu      /* slot not in use */
 
 #define SAMPO_MODE_REFERENCE 0u /* killAgents.cl formulas, initial eggs via initAgents */
@@ -186,6 +187,16 @@ uint64_t sampo_engine_footprint(const sampo_scenario *sc, uint32_t chunk);
 
 /* Free a slot (its instance must not be running in an unsynchronised step). */
 int sampo_engine_release(sampo_engine *e, uint32_t slot);
+
+/* Hold (hold[i] != 0) or resume (hold[i] == 0) the instances in slots
+ * 0..n-1 from the next step on: a running instance becomes SAMPO_HELD and
+ * takes no part in the steps until it is resumed; a held one runs again.
+ * Instances in any other state are left as they are.  Holding changes when
+ * an instance advances, never what it computes: its results stay those of a
+ * run alone.  No counterpart in the reference (one run per process there):
+ * it lets a server advance some admitted scenarios ahead of others, e.g.
+ * the ones with the least work left. */
+int sampo_engine_hold(sampo_engine *e, uint32_t n, const uint8_t *hold);
 
 /* Enqueue n iterations for all running instances; asynchronous. */
 int sampo_engine_step(sampo_engine *e, uint32_t n);

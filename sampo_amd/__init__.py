@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ENGINE_LIB = os.path.join(HERE, "libsampo_engine.so")
 
-RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, GROW, EMPTY = 0, 1, 2, 3, 4, 5, 0xFF
+RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, GROW, HELD, EMPTY = 0, 1, 2, 3, 4, 5, 6, 0xFF
 MODE_REFERENCE, MODE_CONFIG = 0, 1
 STATES = ("egg", "larva", "pupa", "immature", "mateseeking", "bms", "bmd", "gravid")
 
@@ -97,6 +97,7 @@ def lib():
         L.sampo_engine_set_grid.argtypes = [vp, C.c_uint32]
         L.sampo_engine_admit.argtypes = [vp, C.POINTER(Scenario), C.POINTER(C.c_uint32)]
         L.sampo_engine_release.argtypes = [vp, C.c_uint32]
+        L.sampo_engine_hold.argtypes = [vp, C.c_uint32, vp]
         L.sampo_engine_footprint.argtypes = [C.POINTER(Scenario), C.c_uint32]
         L.sampo_engine_footprint.restype = C.c_uint64
         L.sampo_engine_poll.argtypes = [vp, C.c_uint32, vp, vp, vp]
@@ -337,6 +338,13 @@ class Engine:
 
     def release(self, slot):
         _check(lib().sampo_engine_release(self.h, slot))
+
+    def hold(self, mask):
+        """Hold (mask[i] true) or resume the instances in slots 0..len(mask)-1
+        from the next step on; only running instances are held and only
+        held ones resumed."""
+        m = np.ascontiguousarray(np.asarray(mask, dtype=bool).astype(np.uint8))
+        _check(lib().sampo_engine_hold(self.h, m.shape[0], m.ctypes.data))
 
     def poll(self, n):
         """(status, step, agents) arrays for slots 0..n-1 in one read."""

@@ -517,6 +517,22 @@ __global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *p
 	plan_all(inst, ctx, pl_base, pl_par, ninst);
 }
 
+/* Hold (hold[i] != 0) or resume instances 0..n-1, then plan the next pass.
+ * Only running instances are held and only held ones resumed. */
+__global__ void __launch_bounds__(1024) k_hold(Inst *inst, const uint8_t *hold, uint32_t n, Ctx *ctx,
+                                               uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst)
+{
+	for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
+		const uint32_t st = __hip_atomic_load(&inst[i].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		uint32_t to = st;
+		if (hold[i] && st == SAMPO_RUNNING) to = SAMPO_HELD;
+		if (!hold[i] && st == SAMPO_HELD) to = SAMPO_RUNNING;
+		if (to != st) __hip_atomic_store(&inst[i].status, to, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	}
+	__syncthreads();
+	plan_all(inst, ctx, pl_base, pl_par, ninst);
+}
+
 /* ------------------------------------------------------------ k_prep */
 
 #define PREP_WG 1024
@@ -1490,10 +1506,16 @@ struct sampo_engine {
 	int device = 0;
 	hipStream_t stream = nullptr;
 	uint32_t max_slots = 0;
+	/* the launches (pass, prep, plan) cover slots 0..nslots-1: at least one
+	 * past the last slot in use.  It grows at admission and shrinks at a
+	 * release that halves it; either change drops the captured graphs,
+	 * whose launches hold it. */
+	uint32_t nslots = 1;
 	Inst *d_inst = nullptr;
 	Ctx *d_ctx = nullptr;
 	uint32_t *d_pl_base = nullptr;
 	uint8_t *d_pl_par = nullptr;
+	uint8_t *d_hold = nullptr; /* sampo_engine_hold's mask */
 	std::vector<Slot> slots;
 	uint32_t wg = 512, apt = 1, waves = 6;
 	uint32_t chunk = 4096;
@@ -1560,6 +1582,7 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 	HIPCHK(hipMalloc(&e->d_ctx, sizeof(Ctx)));
 	HIPCHK(hipMalloc(&e->d_pl_base, sizeof(uint32_t) * (max_slots + 1)));
 	HIPCHK(hipMalloc(&e->d_pl_par, max_slots));
+	HIPCHK(hipMalloc(&e->d_hold, max_slots));
 	HIPCHK(hipMemset(e->d_ctx, 0, sizeof(Ctx)));
 	std::vector<Inst> blank(max_slots);
 	memset(blank.data(), 0, sizeof(Inst) * max_slots);
@@ -1590,6 +1613,7 @@ extern "C" void sampo_engine_destroy(sampo_engine *e)
 	(void)hipFree(e->d_ctx);
 	(void)hipFree(e->d_pl_base);
 	(void)hipFree(e->d_pl_par);
+	(void)hipFree(e->d_hold);
 	if (e->stream) (void)hipStreamDestroy(e->stream);
 	delete e;
 }
@@ -1833,8 +1857,12 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		free_slot(S);
 		return fail(SAMPO_E_HIP, "upload failed: %s%ld", hipGetErrorString(err));
 	}
+	if (id + 1u > e->nslots) {
+		e->nslots = id + 1u;
+		drop_graph(e);
+	}
 	/* begin iteration 0 on the device (calcStats of src/abms.c:503) */
-	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
+	hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
 		                   e->d_pl_base, e->d_pl_par);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(e->stream));
@@ -1851,12 +1879,32 @@ extern "C" int sampo_engine_release(sampo_engine *e, uint32_t slot)
 	HIPCHK(hipStreamSynchronize(e->stream));
 	uint32_t st = SAMPO_EMPTY;
 	HIPCHK(hipMemcpy(&e->d_inst[slot].status, &st, sizeof(st), hipMemcpyHostToDevice));
+	uint32_t hi = 1; /* one past the last slot still in use */
+	for (uint32_t i = 0; i < e->max_slots; ++i)
+		if (e->slots[i].used && i != slot) hi = i + 1u;
+	if (hi <= e->nslots / 2u) {
+		e->nslots = hi;
+		drop_graph(e);
+	}
 	/* plan the next pass without it before its memory goes */
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
-	                   e->max_slots);
+	                   e->nslots);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(e->stream));
 	free_slot(e->slots[slot]);
+	return 0;
+}
+
+extern "C" int sampo_engine_hold(sampo_engine *e, uint32_t n, const uint8_t *hold)
+{
+	if (!e || (n && !hold) || n > e->max_slots) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	if (n == 0) return 0;
+	HIPCHK(hipMemcpyAsync(e->d_hold, hold, n, hipMemcpyHostToDevice, e->stream));
+	hipLaunchKernelGGL(k_hold, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_hold, n, e->d_ctx, e->d_pl_base,
+	                   e->d_pl_par, e->nslots);
+	HIPCHK(hipGetLastError());
+	HIPCHK(hipStreamSynchronize(e->stream));
 	return 0;
 }
 
@@ -1907,11 +1955,14 @@ extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *g
 	if (!e || !(headroom >= 1.0f)) return fail(SAMPO_E_ARG, "headroom must be >= 1%s%ld");
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
-	std::vector<Inst> h(e->max_slots);
-	HIPCHK(hipMemcpy(h.data(), e->d_inst, sizeof(Inst) * e->max_slots, hipMemcpyDeviceToHost));
+	uint32_t hi = 0; /* slots past the last one in use are not read */
+	for (uint32_t i = 0; i < e->max_slots; ++i)
+		if (e->slots[i].used) hi = i + 1u;
+	std::vector<Inst> h(hi);
+	if (hi) HIPCHK(hipMemcpy(h.data(), e->d_inst, sizeof(Inst) * hi, hipMemcpyDeviceToHost));
 	uint32_t count = 0, resumed = 0;
 	int rc = 0;
-	for (uint32_t i = 0; i < e->max_slots && !rc; ++i) {
+	for (uint32_t i = 0; i < hi && !rc; ++i) {
 		if (!e->slots[i].used) continue;
 		Inst &I = h[i];
 		const uint64_t n = I.st[I.step & 1u].n;
@@ -1932,7 +1983,7 @@ extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *g
 		++count;
 	}
 	if (resumed) {
-		hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
+		hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
 		                   e->d_pl_base, e->d_pl_par);
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipStreamSynchronize(e->stream));
@@ -1950,10 +2001,10 @@ static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after
 {
 	const unsigned int ef = capturing ? hipEventRecordExternal : hipEventRecordDefault;
 	if (before_pass) HIPCHK(hipEventRecordWithFlags(before_pass, e->stream, ef));
-	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->max_slots};
+	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->nslots};
 	HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
 	if (after_pass) HIPCHK(hipEventRecordWithFlags(after_pass, e->stream, ef));
-	hipLaunchKernelGGL(k_prep, dim3(e->max_slots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
+	hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
 	                   e->d_pl_par);
 	return 0;
 }

@@ -6,10 +6,15 @@ Continuous batching at day granularity:
   2. from the head of the queue, requests are admitted into free engine
      slots while their device footprint (sampo_engine_footprint) fits in
      the free device memory less a reserve -- admission by HBM footprint;
-  3. every running instance advances one day (one HIP graph of 24
-     iterations of the batched step), whatever its own iteration count: each
-     instance keeps its own step, horizon, seeds and inputs, so its results
-     are those of a run alone;
+  3. running instances advance one day (one HIP graph of 24 iterations of
+     the batched step), whatever their own iteration counts: each instance
+     keeps its own step, horizon, seeds and inputs, so its results are those
+     of a run alone.  With policy 'srpt' (the default) only the instances
+     with the least work left (agents now x iterations left) advance, as
+     many as make up min_active agents; the others are held
+     (sampo_engine_hold) for that day.  Small requests then finish first
+     instead of all requests sharing every day; with policy 'all' every
+     running instance advances every day;
   4. instances that outgrew their buffers are grown (sampo_engine_service);
      finished ones are read out and their slots and memory released.
 A request's completion latency is the time from its arrival to the end of
@@ -57,7 +62,11 @@ def rows_digest(rows):
 
 class Server:
     def __init__(self, max_slots=1024, reserve_bytes=4 << 30, headroom=1.25, phys_factor=1.25, tuning=None,
-                 keep_rows=False, threads=16, log=None):
+                 keep_rows=False, threads=16, log=None, policy="srpt", min_active=64_000_000):
+        if policy not in ("srpt", "all"):
+            raise ValueError(f"unknown policy {policy!r}")
+        self.policy = policy
+        self.min_active = min_active
         self.max_slots = max_slots
         self.reserve = reserve_bytes
         self.headroom = headroom
@@ -74,6 +83,8 @@ class Server:
         results = [Result(i, specs[i].n0, specs[i].days, specs[i].arrival) for i in range(len(specs))]
         queue = []          # (index, future) in arrival order
         running = {}        # slot -> index
+        fresh = set()       # slots admitted since the last poll
+        last = None         # (status, step, agents) of that poll
         nxt = 0
         rounds = 0
         days_stepped = 0
@@ -115,6 +126,7 @@ class Server:
                     free -= need
                     queue.pop(0)
                     running[slot] = i
+                    fresh.add(slot)
                     results[i].admitted = now()
                 if not running:
                     if nxt >= len(order) and not queue:
@@ -127,6 +139,8 @@ class Server:
                     else:
                         rounds += 1
                     continue
+                if self.policy == "srpt":
+                    self._choose(e, running, specs, last, fresh)
                 e.step_days(1)
                 days_stepped += 1
                 try:
@@ -135,7 +149,8 @@ class Server:
                     if self.log:
                         self.log(f"serve: growth deferred ({err})")
                 rounds += 1
-                st, step, _ = e.poll(self.max_slots)
+                st, step, na = e.poll(max(running) + 1)  # slots past the last running one are free
+                last, fresh = (st, step, na), set()
                 t = now()
                 for slot in [s for s in running if st[s] in FINISHED]:
                     i = running.pop(slot)
@@ -160,6 +175,30 @@ class Server:
             summary.update(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
                            mean=float(lat.mean()), max=float(lat.max()))
         return results, summary
+
+
+    def _choose(self, e, running, specs, last, fresh):
+        """Hold all but the instances with the least work left (agents now
+        x iterations left), keeping at least min_active agents in the step.
+        The state is the poll after the last step; instances admitted since
+        are at iteration 0 with their initial population."""
+        cur = {}
+        for s, i in running.items():
+            if s in fresh or last is None or s >= last[0].shape[0]:
+                cur[s] = (S.RUNNING, 0, specs[i].n0)
+            else:
+                cur[s] = (int(last[0][s]), int(last[1][s]), int(last[2][s]))
+        slots = [s for s in running if cur[s][0] in (S.RUNNING, S.HELD)]
+        left = {s: cur[s][2] * max(specs[running[s]].days * 24 - cur[s][1], 1) for s in slots}
+        na = {s: cur[s][2] for s in slots}
+        mask = np.zeros(max(running) + 1, bool)
+        agents = 0
+        for k, s in enumerate(sorted(slots, key=lambda s: (left[s], s))):
+            if k and agents >= self.min_active:
+                mask[s] = True
+            else:
+                agents += int(na[s])
+        e.hold(mask)
 
 
 def run_alone(spec, phys_factor=0.0, tuning=None):
