@@ -96,18 +96,70 @@ __device__ inline uint64_t count8(const uint32_t (&kw)[4])
 	return a;
 }
 
-/* 8-bit fields of keys 0-3 / 4-7 spread to 16-bit fields */
-__device__ inline void widen(uint64_t c8, uint64_t &ca, uint64_t &cb)
+/* Counting with byte permutes.  For the key pair (2m, 2m + 1) the 8-byte
+ * table {perm_hi(m):perm_lo(m)} holds 0x01 at byte 2m and 0x10 at byte
+ * 2m + 1, zeros elsewhere: v_perm_b32 with a word of 4 keys as the selector
+ * gives, per byte, that key's increment of the pair's two nibble counters --
+ * 4 keys counted by one instruction per pair.  Selector bytes 8-11 read the
+ * sign bits of table bytes (0 here) and 12 reads 0, so keys 8-12 count
+ * nowhere; 13 and more would read 0xff, so the words of a thread that holds
+ * such a key are counted by count8 instead (big_keys). */
+__device__ inline uint32_t perm_lo(uint32_t m) { return m == 0u ? 0x00001001u : (m == 1u ? 0x10010000u : 0u); }
+__device__ inline uint32_t perm_hi(uint32_t m) { return m == 2u ? 0x00001001u : (m == 3u ? 0x10010000u : 0u); }
+
+/* nonzero if a byte of w is 13 or more: (b & 0x7f) + 0x73 carries into the
+ * top bit exactly for 13 <= b < 128, and stays inside its byte */
+__device__ inline uint32_t big_keys(uint32_t w) { return (((w & 0x7f7f7f7fu) + 0x73737373u) | w) & 0x80808080u; }
+
+/* the 4 keys of w into the nibble counters acc (each counter takes one key
+ * per word: 15 words at most between flushes) */
+__device__ inline void nib_add(uint32_t w, uint32_t (&acc)[4])
 {
-	const uint32_t lo = (uint32_t)c8, hi = (uint32_t)(c8 >> 32);
-	ca = ((uint64_t)__builtin_amdgcn_perm(0u, lo, 0x0c030c02u) << 32) | __builtin_amdgcn_perm(0u, lo, 0x0c010c00u);
-	cb = ((uint64_t)__builtin_amdgcn_perm(0u, hi, 0x0c030c02u) << 32) | __builtin_amdgcn_perm(0u, hi, 0x0c010c00u);
+#pragma unroll
+	for (uint32_t m = 0; m < 4u; ++m) acc[m] += __builtin_amdgcn_perm(perm_hi(m), perm_lo(m), w);
 }
 
-/* counts of keys 0-3 and 4-7 among 16 keys, as 16-bit fields */
-__device__ inline void count_keys(const uint32_t (&kw)[4], uint64_t &ca, uint64_t &cb)
+/* the nibble counters summed over their 4 bytes into c[8], then cleared */
+__device__ inline void nib_flush(uint32_t (&acc)[4], uint32_t (&c)[CK])
 {
-	widen(count8(kw), ca, cb);
+#pragma unroll
+	for (uint32_t m = 0; m < 4u; ++m) {
+		c[2u * m] = __builtin_amdgcn_sad_u8(acc[m] & 0x0f0f0f0fu, 0u, c[2u * m]);
+		c[2u * m + 1u] = __builtin_amdgcn_sad_u8((acc[m] >> 4) & 0x0f0f0f0fu, 0u, c[2u * m + 1u]);
+		acc[m] = 0u;
+	}
+}
+
+/* counts c[0..3] / c[4..7] as the 16-bit fields of one word each */
+__device__ inline void pack16(const uint32_t (&c)[CK], uint64_t &ca, uint64_t &cb)
+{
+	ca = ((uint64_t)(c[2] | (c[3] << 16)) << 32) | (c[0] | (c[1] << 16));
+	cb = ((uint64_t)(c[6] | (c[7] << 16)) << 32) | (c[4] | (c[5] << 16));
+}
+
+/* counts of keys 0-3 and 4-7 among the R x 16 keys of kw, as 16-bit fields
+ * (the counters are flushed every 2 rounds: 8 keys each at most) */
+template <uint32_t R>
+__device__ inline void count_keys(const uint32_t (&kw)[R][4], uint64_t &ca, uint64_t &cb)
+{
+	uint32_t c[CK] = {0, 0, 0, 0, 0, 0, 0, 0}, acc[4] = {0, 0, 0, 0}, big = 0;
+#pragma unroll
+	for (uint32_t r = 0; r < R; ++r) {
+#pragma unroll
+		for (uint32_t q = 0; q < 4u; ++q) {
+			nib_add(kw[r][q], acc);
+			big |= big_keys(kw[r][q]);
+		}
+		if (r % 2u == 1u || r + 1u == R) nib_flush(acc, c);
+	}
+	if (big) { /* a key of 13 or more: the exact count */
+		uint64_t c8 = 0; /* at most 64 of a key: 8-bit fields hold it */
+#pragma unroll 1
+		for (uint32_t r = 0; r < R; ++r) c8 += count8(kw[r]); /* rolled: this rare path sets no register budget */
+#pragma unroll
+		for (uint32_t k = 0; k < CK; ++k) c[k] = (uint32_t)(c8 >> (8u * k)) & 0xffu;
+	}
+	pack16(c, ca, cb);
 }
 
 /* phase 1: tile_count[tile * 8 + k] = keys equal to k in the tile; a
@@ -118,26 +170,19 @@ __global__ void __launch_bounds__(CT) k_count(const uint8_t *__restrict__ keys, 
 {
 	__shared__ uint64_t s_w[CCNT][CT / 64u][2];
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-	uint64_t ta[CCNT], tb[CCNT];
-#pragma unroll
-	for (uint32_t g = 0; g < CCNT; ++g) {
+#pragma unroll 1
+	for (uint32_t g = 0; g < CCNT; ++g) { /* one tile at a time: its 16 key words, counts, wave totals */
 		const uint64_t t0 = ((uint64_t)blockIdx.x * CCNT + g) * CTILE;
-		uint64_t c8 = 0; /* at most 64 of a key per thread and tile: 8-bit fields hold it */
+		uint32_t kw[CR][4];
 #pragma unroll
-		for (uint32_t r = 0; r < CR; ++r) {
-			uint32_t kw[4];
-			load_keys(keys, n, t0, r, tid, vec_in, kw);
-			c8 += count8(kw);
-		}
-		widen(c8, ta[g], tb[g]);
-	}
-#pragma unroll
-	for (uint32_t g = 0; g < CCNT; ++g) { /* wave totals; fields stay below 2^16 */
-		ta[g] = wave_incl_scan_fields(ta[g]);
-		tb[g] = wave_incl_scan_fields(tb[g]);
+		for (uint32_t r = 0; r < CR; ++r) load_keys(keys, n, t0, r, tid, vec_in, kw[r]);
+		uint64_t ta, tb;
+		count_keys<CR>(kw, ta, tb);
+		ta = wave_incl_scan_fields(ta); /* fields stay below 2^16 */
+		tb = wave_incl_scan_fields(tb);
 		if (lane == 63u) {
-			s_w[g][w][0] = ta[g];
-			s_w[g][w][1] = tb[g];
+			s_w[g][w][0] = ta;
+			s_w[g][w][1] = tb;
 		}
 	}
 	__syncthreads();
@@ -238,6 +283,7 @@ __global__ void __launch_bounds__(CT) k_rank(const uint8_t *__restrict__ keys, u
 	__shared__ uint64_t s_w[CT / 64u][CR][2];
 	__shared__ uint64_t s_ea[CR][CT], s_eb[CR][CT]; /* per round: each thread's exclusive counts in the tile */
 	__shared__ uint32_t s_pref[CK];
+	__shared__ uint16_t s_base[CK][CT]; /* per key: this thread's exclusive count in the tile, this round */
 	__shared__ uint32_t s_out[CSUB + CSUB / 32u];
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
 	const uint64_t t0 = (uint64_t)blockIdx.x * CTILE;
@@ -252,7 +298,8 @@ __global__ void __launch_bounds__(CT) k_rank(const uint8_t *__restrict__ keys, u
 #pragma unroll 1
 	for (uint32_t r = 0; r < CR; ++r) {
 		uint64_t ca, cb;
-		count_keys(kw[r], ca, cb);
+		uint32_t one[1][4] = {{kw[r][0], kw[r][1], kw[r][2], kw[r][3]}};
+		count_keys<1>(one, ca, cb);
 		const uint64_t xa = wave_incl_scan_fields(ca), xb = wave_incl_scan_fields(cb);
 		if (lane == 63u) {
 			s_w[w][r][0] = xa;
@@ -281,19 +328,25 @@ __global__ void __launch_bounds__(CT) k_rank(const uint8_t *__restrict__ keys, u
 		rb += tb;
 	}
 	/* per round: ranks into LDS in key order, then out in contiguous
-	 * 16-byte pieces (a wave stores 1 KiB per instruction) */
+	 * 16-byte pieces (a wave stores 1 KiB per instruction).  A key's rank is
+	 * its tile's prefix, plus this thread's exclusive count of it in the
+	 * tile (read from LDS by the key), plus its earlier copies among the
+	 * thread's 16 keys of the round (nibble counters in one register: at
+	 * most 15 before the 16th key). */
 #pragma unroll
 	for (uint32_t r = 0; r < CR; ++r) {
-		uint64_t xa = s_ea[r][tid], xb = s_eb[r][tid];
+		const uint64_t xa = s_ea[r][tid], xb = s_eb[r][tid];
+#pragma unroll
+		for (uint32_t k = 0; k < CK; ++k) s_base[k][tid] = (uint16_t)field16(k < 4u ? xa : xb, k & 3u);
+		uint32_t run = 0;
 #pragma unroll
 		for (uint32_t j = 0; j < CV; ++j) {
-			const uint32_t k = (kw[r][j >> 2] >> (8u * (j & 3u))) & 0xffu;
-			const bool lo = k < 4u, hi = k - 4u < 4u;
-			const uint64_t one = 1ull << (16u * (k & 3u));
-			const uint32_t d = s_pref[k & 7u] + field16(lo ? xa : xb, k & 3u);
-			xa += lo ? one : 0ull;
-			xb += hi ? one : 0ull;
-			s_out[lds_slot(tid * CV + j)] = lo || hi ? d : SAMPO_COMPACT_NONE;
+			const uint32_t k = __builtin_amdgcn_ubfe(kw[r][j >> 2], 8u * (j & 3u), 8u);
+			const uint32_t sh = (k & 7u) << 2;
+			const bool valid = k < 8u;
+			const uint32_t d = s_pref[k & 7u] + s_base[k & 7u][tid] + __builtin_amdgcn_ubfe(run, sh, 4u);
+			run += valid ? 1u << sh : 0u;
+			s_out[lds_slot(tid * CV + j)] = valid ? d : SAMPO_COMPACT_NONE;
 		}
 		__syncthreads();
 #pragma unroll
