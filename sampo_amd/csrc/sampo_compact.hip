@@ -45,7 +45,7 @@
 #define CTILE (CSUB * CR)       /* keys per tile: 16384 */
 #define CK SAMPO_COMPACT_KEYS   /* ranked keys */
 #define CBLK (CT * 4u)          /* tiles per block of the tile scan: 1024 */
-#define CCNT 2u                 /* tiles per work-group of k_count */
+#define CCNT 1u                 /* tiles per work-group of k_count */
 
 namespace {
 
@@ -163,8 +163,8 @@ __device__ inline void count_keys(const uint32_t (&kw)[R][4], uint64_t &ca, uint
 }
 
 /* phase 1: tile_count[tile * 8 + k] = keys equal to k in the tile; a
- * work-group counts CCNT consecutive tiles (fewer, longer work-groups: a
- * work-group per tile spent more time starting than reading) */
+ * work-group counts CCNT consecutive tiles (one: with all its 16 key words
+ * loaded at once, a work-group per tile reads fastest) */
 __global__ void __launch_bounds__(CT) k_count(const uint8_t *__restrict__ keys, uint32_t n, uint32_t ntiles,
                                               uint32_t *__restrict__ tile_count, uint32_t vec_in)
 {
