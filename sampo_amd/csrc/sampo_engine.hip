@@ -236,7 +236,6 @@ struct Ctx {
 	uint32_t ticket;
 	uint32_t total;
 	unsigned long long agent_steps;
-	uint32_t prep_done;    /* k_prep work-groups finished in the current launch */
 };
 
 /* ------------------------------------------------------------ the agent */
@@ -451,10 +450,9 @@ __device__ static inline uint32_t wave_sum(uint32_t v)
 
 /* Lay every running instance's chunks end to end for the next pass:
  * pl_base[i] = first ticket of instance i, ctx->total = all chunks; reset
- * the ticket.  Run by one work-group of 1024 threads: the last k_prep
- * work-group to finish, or k_plan after an instance is released.  Other
- * work-groups wrote the instances, so their fields are read with atomic
- * loads at agent scope. */
+ * the ticket.  Run by one work-group of 1024 threads: k_plan after the
+ * prep of several slots, after a release, or k_hold.  The instances' fields
+ * are read with atomic loads at agent scope. */
 __device__ static void plan_all(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst)
 {
 	__shared__ uint32_t s_scan[1024];
@@ -912,12 +910,23 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 }
 
 /* The instance record is worked on in LDS: the bookkeeping is a chain of
- * dependent scalar updates that would otherwise each wait on global memory. */
+ * dependent scalar updates that would otherwise each wait on global memory.
+ * One work-group per slot.  A single slot plans the next pass from its LDS
+ * record; with more, k_plan follows as its own launch (the launch boundary
+ * publishes every work-group's record: no fences, no counter).  Work-groups
+ * of slots with nothing to fold or begin (empty, held, finished) leave after
+ * reading three words. */
 __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par)
 {
 	__shared__ Inst sI;
-	__shared__ uint32_t s_fold, s_begin, s_par, s_last;
+	__shared__ uint32_t s_fold, s_begin, s_par, s_work;
 	Inst *G = insts + blockIdx.x;
+	if (gridDim.x > 1u) {
+		if (threadIdx.x == 0)
+			s_work = (G->status == SAMPO_RUNNING && G->pending) || G->needs_begin;
+		lds_barrier();
+		if (!s_work) return;
+	}
 	{
 		const uint32_t *src = (const uint32_t *)G;
 		uint32_t *dst = (uint32_t *)&sI;
@@ -954,30 +963,14 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 		uint32_t *dst = (uint32_t *)G;
 		for (uint32_t q = threadIdx.x; q < sizeof(Inst) / 4u; q += PREP_WG) dst[q] = src[q];
 	}
-	if (gridDim.x == 1u) { /* alone: the next pass is planned from the LDS record */
-		if (threadIdx.x == 0) {
-			const uint32_t par = sI.step & 1u;
-			const uint32_t nc = sI.status == SAMPO_RUNNING && sI.st[par].active ? sI.st[par].nchunk : 0u;
-			pl_par[0] = (uint8_t)par;
-			pl_base[0] = 0u;
-			pl_base[1] = nc;
-			ctx->total = nc;
-			ctx->ticket = 0u;
-		}
-		return;
-	}
-	/* the last work-group to finish plans the next pass; every store of the
-	 * copy-out is complete (__syncthreads) and published (fence) first */
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		__threadfence();
-		s_last = atomicAdd(&ctx->prep_done, 1u) == gridDim.x - 1u;
-	}
-	__syncthreads();
-	if (s_last) {
-		__threadfence();
-		plan_all(insts, ctx, pl_base, pl_par, gridDim.x);
-		if (threadIdx.x == 0) ctx->prep_done = 0u;
+	if (gridDim.x == 1u && threadIdx.x == 0) { /* alone: the next pass is planned from the LDS record */
+		const uint32_t par = sI.step & 1u;
+		const uint32_t nc = sI.status == SAMPO_RUNNING && sI.st[par].active ? sI.st[par].nchunk : 0u;
+		pl_par[0] = (uint8_t)par;
+		pl_base[0] = 0u;
+		pl_base[1] = nc;
+		ctx->total = nc;
+		ctx->ticket = 0u;
 	}
 }
 
@@ -1595,6 +1588,16 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 	return 0;
 }
 
+/* k_prep over the slots in use; with more than one, k_plan after it */
+static void enqueue_prep(sampo_engine *e)
+{
+	hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
+	                   e->d_pl_par);
+	if (e->nslots > 1u)
+		hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
+		                   e->nslots);
+}
+
 static void free_slot(Slot &s)
 {
 	if (s.mem) (void)hipFree(s.mem);
@@ -1862,8 +1865,7 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		drop_graph(e);
 	}
 	/* begin iteration 0 on the device (calcStats of src/abms.c:503) */
-	hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
-		                   e->d_pl_base, e->d_pl_par);
+	enqueue_prep(e);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(e->stream));
 	S.used = true;
@@ -1983,8 +1985,7 @@ extern "C" int sampo_engine_service(sampo_engine *e, float headroom, uint32_t *g
 		++count;
 	}
 	if (resumed) {
-		hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx,
-		                   e->d_pl_base, e->d_pl_par);
+		enqueue_prep(e);
 		HIPCHK(hipGetLastError());
 		HIPCHK(hipStreamSynchronize(e->stream));
 	}
@@ -2004,8 +2005,7 @@ static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after
 	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->nslots};
 	HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
 	if (after_pass) HIPCHK(hipEventRecordWithFlags(after_pass, e->stream, ef));
-	hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
-	                   e->d_pl_par);
+	enqueue_prep(e);
 	return 0;
 }
 
