@@ -127,30 +127,36 @@ struct Chunk {
 };
 
 /* How an iteration's dense range is cut into chunks: nbig chunks of `big`
- * agents, then chunks of `small` agents (equal chunks: nbig = 0).  With few
- * chunks per work-group, small last chunks let the persistent work-groups
- * of the pass finish together: one that is done early takes another small
- * one instead of the last big one holding the whole launch. */
+ * agents, nmid of `mid`, then chunks of `small` agents (equal chunks: nbig =
+ * nmid = 0).  With few chunks per work-group, smaller last chunks let the
+ * persistent work-groups of the pass finish together: one that is done
+ * early takes another small one instead of the last big one holding the
+ * whole launch. */
 struct Cut {
-	uint32_t big, small, nbig;
+	uint32_t big, mid, small, nbig, nmid;
 };
 
 __host__ __device__ static inline uint32_t cut_start(const Cut &k, uint32_t c)
 {
-	return c <= k.nbig ? c * k.big : k.nbig * k.big + (c - k.nbig) * k.small;
+	if (c <= k.nbig) return c * k.big;
+	const uint32_t b1 = k.nbig * k.big;
+	if (c <= k.nbig + k.nmid) return b1 + (c - k.nbig) * k.mid;
+	return b1 + k.nmid * k.mid + (c - k.nbig - k.nmid) * k.small;
 }
 
 /* the first chunk starting at or after dense index x; cut_at_or_after(k, N)
  * is the number of chunks covering N agents */
 __host__ __device__ static inline uint32_t cut_at_or_after(const Cut &k, uint32_t x)
 {
-	const uint32_t b = k.nbig * k.big;
-	return x <= b ? (x + k.big - 1u) / k.big : k.nbig + (x - b + k.small - 1u) / k.small;
+	const uint32_t b1 = k.nbig * k.big, b2 = b1 + k.nmid * k.mid;
+	if (x <= b1) return (x + k.big - 1u) / k.big;
+	if (x <= b2) return k.nbig + (x - b1 + k.mid - 1u) / k.mid;
+	return k.nbig + k.nmid + (x - b2 + k.small - 1u) / k.small;
 }
 
 __host__ __device__ static inline bool cut_equal(const Cut &a, const Cut &b)
 {
-	return a.big == b.big && a.small == b.small && a.nbig == b.nbig;
+	return a.big == b.big && a.mid == b.mid && a.small == b.small && a.nbig == b.nbig && a.nmid == b.nmid;
 }
 
 /* Output layout.  The agents of chunk c that had state s at the start of the
@@ -535,31 +541,32 @@ __global__ void __launch_bounds__(1024) k_hold(Inst *inst, const uint8_t *hold, 
 
 #define PREP_WG 1024
 
-/* chunk size of an iteration with N agents: at least chunk_size, and at
- * most target_chunks chunks */
 /* Chunks of N agents: at least chunk_size agents each, about target_chunks
  * of them.  When that makes chunks of 4 x chunk_size or more, the first
- * target/2 chunks take 80 % of the agents and the rest are chunks of a
- * quarter of their size. */
+ * target/2 chunks take 80 % of the agents; of the rest, target/8 chunks are
+ * of one size and the others of half that size (the last ones handed out
+ * are the smallest). */
 __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_t N)
 {
 	const uint32_t cs = I->chunk_size, T = I->target_chunks;
-	Cut k = {cs, cs, 0u};
+	Cut k = {cs, cs, cs, 0u, 0u};
 	if (!T) return k;
 	const uint32_t want = sm_roundup((uint32_t)(((uint64_t)N + T - 1u) / T), 256u);
 	if (want <= cs) return k;
-	k.big = k.small = want;
-	if (want >= 4u * cs && T >= 4u) {
-		const uint32_t nb = T / 2u;
+	k.big = k.mid = k.small = want;
+	if (want >= 4u * cs && T >= 8u) {
+		const uint32_t nb = T / 2u, n1 = T / 8u, n2 = T - nb - n1;
+		const uint32_t big = sm_roundup((uint32_t)(((uint64_t)N * 4u + 5u * nb - 1u) / (5u * (uint64_t)nb)), 256u);
+		if ((uint64_t)nb * big >= N) return k; /* rounding left no tail: equal chunks */
+		const uint64_t rest = N - (uint64_t)nb * big;
+		/* n1 chunks of m and n2 of m / 2 cover rest: m = rest / (n1 + n2 / 2) */
+		uint32_t m = sm_roundup((uint32_t)((2u * rest + 2u * n1 + n2 - 1u) / (2u * (uint64_t)n1 + n2)), 512u);
+		if (m < 2u * cs) m = 2u * cs;
+		k.big = big;
 		k.nbig = nb;
-		k.big = sm_roundup((uint32_t)(((uint64_t)N * 4u + 5u * nb - 1u) / (5u * (uint64_t)nb)), 256u);
-		const uint32_t rest = N > nb * k.big ? N - nb * k.big : 0u;
-		const uint32_t sm = sm_roundup((uint32_t)(((uint64_t)rest + (T - nb) - 1u) / (T - nb)), 256u);
-		k.small = sm > cs ? sm : cs;
-		if (nb * (uint64_t)k.big >= N) { /* rounding left no tail: equal chunks */
-			k.big = k.small = want;
-			k.nbig = 0u;
-		}
+		k.mid = m;
+		k.nmid = n1;
+		k.small = m / 2u;
 	}
 	return k;
 }
