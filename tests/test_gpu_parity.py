@@ -237,6 +237,30 @@ def test_engine_many_instances_match_solo_runs():
             agents_equal(e.export(s)[1], res[0][i][1])
 
 
+@pytest.mark.parametrize("target", [8, 16, 64, 200])
+def test_engine_chunks_of_three_sizes_match_oracle(target):
+    """A small target chunk count over a minimum of 64 agents: half the
+    chunks big with 80 % of the agents, an eighth of one size, the rest half
+    that size (or equal chunks where rounding leaves no tail), as the pass
+    cuts one large run."""
+    n0, days, seed, dmr = 20000, 4, 33, (0.05, 0.03, 0.02, 0.1)
+    steps = 24 * days
+    temp, lay, ag = config_inputs(n0, days, seed)
+    env = S.make_env(cc=0.05 * n0)
+    sim = O.Sim(temp=temp, env=env, capacity=60 * n0, seed=seed, layout=lay, agents=ag, mode=1, dmr=dmr)
+    orows = sim.run(steps)
+    with S.Engine(max_slots=1) as e:
+        e.set_chunk(64, target)
+        slot = e.admit(mode=S.MODE_CONFIG, seed=seed, horizon=steps, temp=temp, env=env, max_agents=60 * n0,
+                       dmr=dmr, layout=lay, agents=ag)
+        e.step(steps)
+        rows_equal(orows, e.rows(slot, 0, steps))
+        lay_o, ag_o, _ = sim.export()
+        lay_g, ag_g = e.export(slot)
+        assert np.array_equal(lay_o, lay_g)
+        agents_equal(ag_o, ag_g)
+
+
 @pytest.mark.parametrize("chunk", [64, 200, 1000])
 def test_engine_small_chunks_match_oracle(chunk):
     """Many chunks per state: blocks split across chunks, tiles that span
