@@ -51,12 +51,12 @@ HBM_PEAK_GBS = 8000.0
 RECORD_BYTES = 34  # canonical agent record: 5 x f32 + 2 x u32 + 2 x u16 + 2 x u8
 
 WORKLOADS = {
-    # configs[1] of the benchmark definition
+    # the headline workload: one large run
     "large": dict(
         desc="plain single run, large: 32M initial agents, buffer capacity 128M, 365-day horizon, "
              "config-mode mortality 0.05/0.03/0.02/0.1 per day, T(d)=25+5sin(2pi d/365)+N(0,1.5)",
         n0=32_000_000, max_agents=128_000_000, cc_per_agent=0.025, seed=1, days=365),
-    # configs[0]
+    # one small run, the size a CPU runs in minutes
     "small": dict(
         desc="plain single run, small: 100k initial agents, 365-day horizon, config-mode mortality",
         n0=100_000, max_agents=16_000_000, cc_per_agent=0.025, seed=1, days=365),
