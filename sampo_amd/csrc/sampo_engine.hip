@@ -121,8 +121,8 @@ __host__ __device__ static inline uint32_t seg_slot(uint32_t pbase, uint32_t sta
 }
 
 struct Chunk {
-	uint32_t k0, k1; /* dense range */
-	uint32_t seg0;   /* segment holding k0 */
+	uint32_t k0, k1; /* dense range: worked out by the pass from the iteration's cut, not stored */
+	uint32_t seg0;   /* segment holding k0: the one field stored per chunk */
 	uint32_t pad;
 };
 
@@ -152,11 +152,6 @@ __host__ __device__ static inline uint32_t cut_at_or_after(const Cut &k, uint32_
 	if (x <= b1) return (x + k.big - 1u) / k.big;
 	if (x <= b2) return k.nbig + (x - b1 + k.mid - 1u) / k.mid;
 	return k.nbig + k.nmid + (x - b2 + k.small - 1u) / k.small;
-}
-
-__host__ __device__ static inline bool cut_equal(const Cut &a, const Cut &b)
-{
-	return a.big == b.big && a.mid == b.mid && a.small == b.small && a.nbig == b.nbig && a.nmid == b.nmid;
 }
 
 /* Output layout.  The agents of chunk c that had state s at the start of the
@@ -799,9 +794,18 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
  * printStats row, checks, per-iteration parameters and chunks. */
 __device__ static void begin_iteration(Inst *I, uint32_t par)
 {
-	__shared__ uint32_t s_status, s_go;
+	__shared__ uint32_t s_status;
 	const uint32_t tid = threadIdx.x;
 	IterState &ip = I->st[par];
+	if (tid >= 128u && tid < 228u) {
+		/* the larval mortality table of this iteration, beside tid 0's work:
+		 * killAgents.cl's larval rate with L1DE and CC (reference mode), or
+		 * the stage rate (unused when the iteration does not run) */
+		const uint32_t d = tid - 128u;
+		const uint32_t t = ip.step < I->horizon ? ip.step : I->horizon - 1u;
+		const float cc = *gmem(&I->env[I->env_count == 1u ? 0u : t].carryingCapacity);
+		ip.hmr_larva[d] = I->mode == SAMPO_MODE_REFERENCE ? sm_hmr_larva(d, I->l1de, cc) : I->hmr_stage[1];
+	}
 	if (tid >= 64u && tid < 88u && ip.step < I->horizon) {
 		/* the seed sets step + 1 (seed_set) and step (nb_set) of a running
 		 * iteration (step < horizon: both exist), loaded beside tid 0's work */
@@ -881,18 +885,14 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 	/* chunks (chunk_cut).  Their outputs go to the same dense slot range of
 	 * the other buffer, so N <= P is all the room needed. */
 	const uint32_t N = ip.n;
-	const Cut cut = chunk_cut(I, N);
+	/* after a fold, its cut of these N agents and each chunk's first segment
+	 * are in place; otherwise both are worked out here */
+	const bool have_seg0 = ip.seg0_done;
+	const Cut cut = have_seg0 ? ip.cut : chunk_cut(I, N);
 	const uint32_t C = st == SAMPO_RUNNING ? cut_at_or_after(cut, N) : 0u;
-	const bool have_seg0 = ip.seg0_done && cut_equal(ip.cut, cut);
 	auto *ch = gmem(I->chunk[par]);
 	const auto *seg = gmem(I->seg[par]);
 	const uint32_t nseg = ip.nseg;
-	for (uint32_t c = tid; c < C; c += PREP_WG) {
-		const uint32_t e = cut_start(cut, c + 1u);
-		ch[c].k0 = cut_start(cut, c);
-		ch[c].k1 = e < N ? e : N;
-	}
-	lds_barrier();
 	/* initial populations: segment holding each chunk's first agent, segment
 	 * j holding the chunks whose start falls in [k0_j, k0_(j+1)) */
 	if (!have_seg0)
@@ -908,12 +908,7 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		ip.nchunk = ip.active ? C : 0u;
 		I->status = st;
 		I->step = ip.step;
-		s_go = st == SAMPO_RUNNING ? 1u + (I->mode == SAMPO_MODE_REFERENCE ? 1u : 0u) : 0u;
 	}
-	lds_barrier();
-	if (s_go)
-		for (uint32_t d = tid; d < 100u; d += PREP_WG)
-			ip.hmr_larva[d] = s_go == 2u ? sm_hmr_larva(d, I->l1de, ip.env.carryingCapacity) : I->hmr_stage[1];
 }
 
 /* The instance record is worked on in LDS: the bookkeeping is a chain of
@@ -1180,7 +1175,12 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			}
 			lds_barrier();
 		}
-		if (tid < 4u) ((uint32_t *)&T.ch)[tid] = ((const gu32 *)(T.chk + c))[tid];
+		if (tid == 0) { /* the chunk's range from the iteration's cut; its first segment from the record */
+			const uint32_t e = cut_start(T.ip.cut, c + 1u);
+			T.ch.k0 = cut_start(T.ip.cut, c);
+			T.ch.k1 = e < T.ip.n ? e : T.ip.n;
+			T.ch.seg0 = *gmem(&T.chk[c].seg0);
+		}
 		if (tid < 16u) s_run[tid] = 0u;
 		lds_barrier();
 		if (tid < 8u) {
