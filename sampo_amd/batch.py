@@ -38,7 +38,7 @@ from sampo_amd import workloads as W
 
 BUCKET_BASE = 1e3
 CHUNK_CANDIDATES = (4096, 16384, 65536, 262144)
-TILE_CANDIDATES = ((512, 1, 6), (256, 1, 8), (512, 2, 0), (1024, 1, 6))
+TILE_CANDIDATES = ((768, 1, 6), (512, 1, 6), (256, 1, 8), (1024, 1, 6))
 DEFAULT_CHUNK = 16384
 NO_CHUNK_CAP = 1 << 24  # target chunks per instance: effectively unlimited
 
@@ -53,7 +53,7 @@ def bucket_range(b):
 
 @dataclass
 class Tuning:
-    tile: tuple = (512, 1, 6)
+    tile: tuple = (768, 1, 6)
     chunk: dict = field(default_factory=dict)  # bucket -> agents per chunk
     table: list = field(default_factory=list)  # measurements
 

@@ -1493,6 +1493,7 @@ pass_fn pick_pass(uint32_t wg, uint32_t apt, uint32_t waves)
 	SAMPO_TILE(256, 1, 0) SAMPO_TILE(256, 2, 0) SAMPO_TILE(256, 4, 0)
 	SAMPO_TILE(256, 1, 6) SAMPO_TILE(256, 1, 8) SAMPO_TILE(256, 2, 6)
 	SAMPO_TILE(512, 1, 0) SAMPO_TILE(512, 2, 0) SAMPO_TILE(512, 1, 6) SAMPO_TILE(512, 1, 8)
+	SAMPO_TILE(768, 1, 6)
 	SAMPO_TILE(1024, 1, 0) SAMPO_TILE(1024, 1, 6)
 #undef SAMPO_TILE
 	return nullptr;
@@ -1517,7 +1518,7 @@ struct sampo_engine {
 	uint8_t *d_pl_par = nullptr;
 	uint8_t *d_hold = nullptr; /* sampo_engine_hold's mask */
 	std::vector<Slot> slots;
-	uint32_t wg = 512, apt = 1, waves = 6;
+	uint32_t wg = 768, apt = 1, waves = 6; /* two work-groups of 12 waves per CU */
 	uint32_t chunk = 4096;
 	uint32_t target_chunks = 0; /* 0: 2 per resident work-group of the pass */
 	pass_fn pass = nullptr;

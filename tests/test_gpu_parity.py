@@ -324,7 +324,7 @@ def test_engine_tile_shapes_agree():
     steps = 24 * 4
     temp, env = ref_inputs(steps, 20000.0)
     ref = None
-    for wg, apt, waves in ((256, 1, 0), (128, 2, 0), (64, 4, 0), (512, 2, 0), (256, 4, 0), (512, 1, 6),
+    for wg, apt, waves in ((256, 1, 0), (128, 2, 0), (64, 4, 0), (512, 2, 0), (256, 4, 0), (512, 1, 6), (768, 1, 6),
                            (1024, 1, 0), (256, 1, 8)):
         with S.Engine(max_slots=1) as e:
             e.set_tile(wg, apt, waves)
