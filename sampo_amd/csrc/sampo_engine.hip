@@ -538,9 +538,10 @@ __global__ void __launch_bounds__(1024) k_hold(Inst *inst, const uint8_t *hold, 
 
 /* Chunks of N agents: at least chunk_size agents each, about target_chunks
  * of them.  When that makes chunks of 4 x chunk_size or more, the first
- * target/2 chunks take 80 % of the agents; of the rest, target/8 chunks are
- * of one size and the others of half that size (the last ones handed out
- * are the smallest). */
+ * target/2 chunks (one per work-group of the pass) take 80 % of the agents;
+ * of the rest, target/8 chunks are of one size and up to 3 x target/4 of
+ * half that size, the last ones handed out being the smallest: the launch
+ * ends about when the last small chunk does. */
 __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_t N)
 {
 	const uint32_t cs = I->chunk_size, T = I->target_chunks;
@@ -550,7 +551,7 @@ __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_
 	if (want <= cs) return k;
 	k.big = k.mid = k.small = want;
 	if (want >= 4u * cs && T >= 8u) {
-		const uint32_t nb = T / 2u, n1 = T / 8u, n2 = T - nb - n1;
+		const uint32_t nb = T / 2u, n1 = T / 8u, n2 = 2u * (T - nb - n1);
 		const uint32_t big = sm_roundup((uint32_t)(((uint64_t)N * 4u + 5u * nb - 1u) / (5u * (uint64_t)nb)), 256u);
 		if ((uint64_t)nb * big >= N) return k; /* rounding left no tail: equal chunks */
 		const uint64_t rest = N - (uint64_t)nb * big;
