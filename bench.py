@@ -219,8 +219,10 @@ def main():
     ap.add_argument("--target-chunks", type=int, default=0,
                     help="single run: most chunks per iteration (0: two per resident work-group)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-days", type=int, default=2)
-    ap.add_argument("--cpu-scale", type=int, default=10)
+    ap.add_argument("--cpu-days", type=int, default=0,
+                    help="CPU baseline: days simulated (0: 2 for the large run, warmup + steps for the small run)")
+    ap.add_argument("--cpu-scale", type=int, default=0,
+                    help="CPU baseline: population divisor (0: 10 for the large run, 1 for the small run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path runs on one GPU, "
@@ -299,7 +301,10 @@ def main():
                             "kernels": "k_count, k_scan_local, k_scan_top, k_rank"},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_days, args.cpu_scale)
+        small = args.workload == "small"
+        days = args.cpu_days or (args.warmup + args.steps if small else 2)
+        scale = args.cpu_scale or (1 if small else 10)
+        out["cpu_baseline"] = cpu_baseline(w, days, scale)
     print(json.dumps(out), flush=True)
 
 
