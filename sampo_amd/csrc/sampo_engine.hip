@@ -246,9 +246,10 @@ struct Ag {
 	uint32_t aux, avail, cyc, misc;
 };
 
-struct Acc {
-	uint32_t eggs, bites, ibites, ncyc, scyc;
-};
+/* the event sums of agent_step, one LDS word each per wave: eggs laid,
+ * bites, infective bites, completed cycles, their summed lengths */
+enum { EV_EGGS, EV_BITES, EV_IBITES, EV_NCYC, EV_SCYC, NEV };
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 /* per-chunk copy of the instance's iteration parameters */
 struct TileParams {
@@ -286,9 +287,16 @@ __device__ __attribute__((noinline)) uint32_t gen_eggs(float u0, float u1, uint3
 	return sm_egg_batch(u0, u1, p[0], p[1], p[2], nb);
 }
 
+__device__ static inline void ev_add(lds_u32 *ev, uint32_t i, uint32_t v)
+{
+	__hip_atomic_fetch_add(&ev[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 /* killAgents + updateAgents for one agent in state s at virtual index g.
- * Returns the new state, or S_DEAD. */
-__device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, const TileParams &T, Acc &acc)
+ * Returns the new state, or S_DEAD.  The rare events add to the wave's LDS
+ * sums ev (registers held across the tile loop for them were spilled, and
+ * every spill reload waited for the next tile's loads in flight). */
+__device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, const TileParams &T, lds_u32 *ev)
 {
 	const IterState &ip = T.ip;
 	/* killAgents.cl:55-147 */
@@ -357,10 +365,10 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 			if (itn > 0.0f && rnd(&T.ip.seed[8], g) <= itn) return S_DEAD;
 			const uint32_t human = ip.bite_human < 1.0f ? (rnd(&T.ip.seed[12], g) <= ip.bite_human) : 1u;
 			a.misc = (a.misc & 0xffff0000u) | ((hbc + human) & 0xffffu);
-			acc.bites += 1u;
+			ev_add(ev, EV_BITES, 1u);
 			a.his = 0.0f;
 			a.dly = ip.bmd_dly;
-			if (a.csd >= 1.0f && human) acc.ibites += 1u;
+			if (a.csd >= 1.0f && human) ev_add(ev, EV_IBITES, 1u);
 			return S_BMD;
 		}
 		return S_BMS;
@@ -392,13 +400,13 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 				float nbm = (float)ip.tb / (((float)a.aux + 1.0f) * ip.env.carryingCapacity);
 				float pbm = sc_maxf(0.0f, 1.0f - nbm);
 				uint32_t pe = sc_f2u(sc_ceilf((float)a.avail * pbm));
-				acc.eggs += pe;
+				ev_add(ev, EV_EGGS, pe);
 				a.avail -= pe;
 			}
 			a.aux += 1u;
 			if (a.avail == 0u) {
-				acc.ncyc += 1u;
-				acc.scyc += a.cyc;
+				ev_add(ev, EV_NCYC, 1u);
+				ev_add(ev, EV_SCYC, a.cyc);
 				a.cyc = 0u;
 				a.his = 0.0f;
 				a.dly = ip.bms_dly;
@@ -1012,11 +1020,13 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	__shared__ uint32_t s_wc[APT][NW][16];    /* per (input state, moved) */
 	__shared__ uint32_t s_run[16], s_tc[16];
 	__shared__ uint32_t s_head[8], s_end[8];  /* the chunk's block of each input state */
-	__shared__ uint32_t s_acc[NW][8];
+	__shared__ uint32_t s_acc[NW][3];   /* l1de, females, potentially infective: per wave */
+	__shared__ uint32_t s_ev[NW][NEV]; /* agent_step's event sums of the chunk, per wave */
 	__shared__ uint32_t bufA[NFIELD][TILE], bufB[NFIELD][TILE];
 	__shared__ uint32_t s_sg[TILE], s_ss[TILE]; /* next tile, per lane: virtual index; state | newborn << 8 */
 
-	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+	const uint32_t tid = threadIdx.x, lane = tid & 63u;
+	const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6); /* scalar: addresses built on it need no VGPR */
 	const uint32_t total = ctx->total;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 	uint32_t next = 0; /* global index of the segment holding the last issued tile's end */
@@ -1183,6 +1193,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			T.ch.seg0 = *gmem(&T.chk[c].seg0);
 		}
 		if (tid < 16u) s_run[tid] = 0u;
+		if (tid < NW * NEV) (&s_ev[0][0])[tid] = 0u;
 		lds_barrier();
 		if (tid < 8u) {
 			uint32_t h = T.ip.kstart[tid] > T.ch.k0 ? T.ip.kstart[tid] : T.ch.k0;
@@ -1191,7 +1202,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			s_end[tid] = z > h ? z : h;
 		}
 		const uint32_t k0 = T.ch.k0, k1 = T.ch.k1;
-		Acc acc = {0, 0, 0, 0, 0};
+		lds_u32 *const ev = (lds_u32 *)&s_ev[w][0]; /* this wave's event sums */
 		uint32_t l1de = 0, fem = 0, pinf = 0;
 
 		/* per-field bases of both buffers in scalar registers */
@@ -1268,7 +1279,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t s = me[jx].s;
 					const uint32_t o = (uint32_t)jx * WG + tid;
 					Ag x = a[jx];
-					const uint32_t ns2 = agent_step(x, s, me[jx].g, T, acc);
+					const uint32_t ns2 = agent_step(x, s, me[jx].g, T, ev);
 					if (ns2 < S_DEAD) {
 						n2 = 2u * s + (ns2 != s ? 1u : 0u);
 						cur[F_AGE][o] = sc_bits_f(x.age);
@@ -1330,16 +1341,16 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			if (ts + TILE < k1) tile(ts + TILE, bufB, bufA);
 		}
 		{ /* chunk results */
-			uint32_t v[8] = {acc.eggs, acc.bites, acc.ibites, acc.ncyc, acc.scyc, l1de, fem, pinf};
+			uint32_t v[3] = {l1de, fem, pinf};
 #pragma unroll
-			for (int q = 0; q < 8; ++q) {
+			for (int q = 0; q < 3; ++q) {
 				uint32_t r = wave_sum(v[q]);
 				if (lane == 0) s_acc[w][q] = r;
 			}
 			lds_barrier();
-			if (tid < 8u) {
+			if (tid < 8u) { /* the sums in cout order: the NEV events, then l1de, fem, pinf */
 				uint32_t r = 0;
-				for (int q = 0; q < NW; ++q) r += s_acc[q][tid];
+				for (int q = 0; q < NW; ++q) r += tid < NEV ? s_ev[q][tid] : s_acc[q][tid - NEV];
 				((gu32 *)T.cout)[(size_t)(CO_SUMS + tid) * T.cmax + c] = r;
 			}
 			if (tid < 16u) /* stayed: row q, moved on: row 8 + q */
