@@ -215,7 +215,9 @@ def main():
                     help="serve, srpt: agents kept in each day's step at least")
     ap.add_argument("--tile", default="768x1x6", help="work-group size x agents per thread x min waves per SIMD")
     ap.add_argument("--per-cu", type=int, default=0, help="persistent work-groups of the pass per CU (0: all that fit)")
-    ap.add_argument("--chunk", type=int, default=4096, help="single run: minimum agents per chunk")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="single run: minimum agents per chunk (0: 4096, or 512 below 1M initial agents, where "
+                         "4096-agent chunks would leave most work-groups idle)")
     ap.add_argument("--target-chunks", type=int, default=0,
                     help="single run: most chunks per iteration (0: two per resident work-group)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -239,6 +241,8 @@ def main():
         sys.exit("warmup + steps exceeds the workload's horizon")
     temp, env, layout, agents = build_inputs(w)
     wg, apt, waves = (int(x) for x in (args.tile.split("x") + ["0"])[:3])
+    if not args.chunk:
+        args.chunk = 512 if w["n0"] < 1_000_000 else 4096
     with S.Engine(max_slots=1) as e:
         e.set_tile(wg, apt, waves)
         e.set_grid(args.per_cu)
