@@ -14,17 +14,18 @@
  *                            once into its input block's slots in the other
  *                            buffer: stayers from the head, movers from the
  *                            tail.  No work-group waits on another;
- *   k_prep  (1 group/instance) folds the chunks' counts and sums into the new
- *                            segment list -- the compaction of preScan + scan
- *                            + oldToNewAgents (kernel/preScan.cl,
- *                            kernel/boltScan.cl, kernel/oldToNewAgents.cl) --
- *                            computes the next ranges, printStats row and
- *                            calcStats sums (kernel/calcL1de.cl,
- *                            kernel/calcGender.cl), the larval mortality
- *                            table, and cuts the next chunks; its last
- *                            work-group to finish lays every running
- *                            instance's chunks end to end for the next
- *                            pass (plan_all).
+ *   k_prep  (1 group per slot in use) folds the chunks' counts and sums
+ *                            into the new segment list -- the compaction of
+ *                            preScan + scan + oldToNewAgents
+ *                            (kernel/preScan.cl, kernel/boltScan.cl,
+ *                            kernel/oldToNewAgents.cl) -- computes the next
+ *                            ranges, printStats row and calcStats sums
+ *                            (kernel/calcL1de.cl, kernel/calcGender.cl), the
+ *                            larval mortality table, and cuts the next
+ *                            chunks.  A single slot plans the next pass from
+ *                            its LDS record; with several, k_plan (one
+ *                            work-group, launched after the prep) lays every
+ *                            running instance's chunks end to end (plan_all).
  *
  * Survivors keep reference order: chunks are in old-index order, each run
  * (chunk c, input state, stayed / moved on) holds its agents in old-index
