@@ -105,6 +105,18 @@ def main(ref):
     out["taus"] = [[int(a) for a in s] for s in steps]
     lcg = re.search(r"LCGStep\(z4, (\d+), (\d+)UL\)", open(gr).read())
     out["lcg"] = [int(lcg.group(1)), int(lcg.group(2))]
+    # how HybridTaus forms each component's state from the seed and the
+    # global id: z_i = seeds->z_i [+ offset] [+ gid] or seeds->z_i * gid
+    gtext = open(gr).read()
+    state = []
+    for m in re.finditer(r"UINT z(\d) = seeds->z\d((?: [+*] (?:\d+u|gid))*);", gtext):
+        terms = re.findall(r"([+*]) (\d+u|gid)", m.group(2))
+        state.append({"z": int(m.group(1)),
+                      "add": sum(int(v.rstrip("u")) for op, v in terms if op == "+" and v != "gid"),
+                      "add_gid": any(op == "+" and v == "gid" for op, v in terms),
+                      "mul_gid": any(op == "*" and v == "gid" for op, v in terms),
+                      "source": f"include/gpuRand.h:{gtext[:m.start()].count(chr(10)) + 1}"})
+    out["hybridState"] = state
     abms = os.path.join(ref, "src", "abms.c")
     txt = open(abms).read()
     out["abms"] = {"FACTOR": int(re.search(r"#define FACTOR (\d+)", txt).group(1)),
