@@ -50,26 +50,12 @@ METRIC = "agent-steps/s batched vs plain on 1 GPU (bit-exact); %HBM peak; p50/p9
 HBM_PEAK_GBS = 8000.0
 RECORD_BYTES = 34  # canonical agent record: 5 x f32 + 2 x u32 + 2 x u16 + 2 x u8
 
-WORKLOADS = {
-    # the headline workload: one large run
-    "large": dict(
-        desc="plain single run, large: 32M initial agents, buffer capacity 128M, 365-day horizon, "
-             "config-mode mortality 0.05/0.03/0.02/0.1 per day, T(d)=25+5sin(2pi d/365)+N(0,1.5)",
-        n0=32_000_000, max_agents=128_000_000, cc_per_agent=0.025, seed=1, days=365),
-    # one small run, the size a CPU runs in minutes
-    "small": dict(
-        desc="plain single run, small: 100k initial agents, 365-day horizon, config-mode mortality",
-        n0=100_000, max_agents=16_000_000, cc_per_agent=0.025, seed=1, days=365),
-}
-DMR = (0.05, 0.03, 0.02, 0.1)
+from sampo_amd.workloads import DMR, SINGLE_RUN as WORKLOADS  # noqa: E402
 
 
 def build_inputs(w):
-    import sampo_amd as S
-    temp = S.gen_temperature(w["seed"], w["days"])
-    layout, agents = S.gen_population(w["seed"], w["n0"], float(temp[0]))
-    env = S.make_env(cc=w["cc_per_agent"] * w["n0"])
-    return temp, env, layout, agents
+    from sampo_amd import workloads as W
+    return W.single_run_inputs(w)
 
 
 def traffic_from_profiles(name):

@@ -18,11 +18,13 @@
 #define SAMPO_ICL_H
 
 #include <stdarg.h>
+#include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -37,6 +39,21 @@ typedef uint64_t cl_ulong;
 typedef uint64_t cl_mem_flags;
 typedef uint64_t cl_device_type;
 typedef uint64_t cl_map_flags;
+
+/* the model's scalar names over them (include/host_types.h:9-12), so that
+ * include/agent.h compiles with this header in place of host_types.h */
+#ifndef REAL
+#define REAL cl_float
+#endif
+#ifndef INT
+#define INT cl_int
+#endif
+#ifndef UINT
+#define UINT cl_uint
+#endif
+#ifndef BOOL
+#define BOOL cl_bool
+#endif
 
 #define CL_FALSE 0u
 #define CL_TRUE 1u

@@ -1723,7 +1723,9 @@ extern "C" uint64_t sampo_engine_footprint(const sampo_scenario *sc, uint32_t ch
 	const uint64_t P = initial_slots(sc, n0);
 	const SlotLayout L = slot_layout(round256(sc->max_agents), chunk ? chunk : 4096u, sc->horizon,
 	                                 sc->env_count ? sc->env_count : 1u);
-	return (uint64_t)L.total + 2ull * NFIELD * 4ull * P;
+	/* plus the staging of the initial population while it is imported */
+	const uint64_t staging = sc->mode == SAMPO_MODE_CONFIG ? n0 * (uint64_t)sizeof(sampo_agent) : 0u;
+	return (uint64_t)L.total + 2ull * NFIELD * 4ull * P + staging;
 }
 
 extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uint32_t *slot_out)
@@ -1799,6 +1801,22 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		stored = k;
 		if (stored > sc->max_agents) return fail(SAMPO_E_ARG, "initial population exceeds max_agents%s%ld");
 	}
+	{ /* humanBloodmealCount and numEggBatches are stored in 16 and 15 bits.
+	     Each grows by at most one per gonotrophic cycle (blood meal seeking ->
+	     digesting -> gravid -> seeking), which takes at least three
+	     iterations since an agent changes state at most once per iteration,
+	     so neither can pass start + ceil(horizon / 3).  Refuse a run in which
+	     they could wrap. */
+		uint64_t hbc0 = 0, nb0 = 0;
+		for (uint64_t i = 0; i < stored; ++i) {
+			if (sc->init_agents[i].humanBloodmealCount > hbc0) hbc0 = sc->init_agents[i].humanBloodmealCount;
+			if (sc->init_agents[i].numEggBatches > nb0) nb0 = sc->init_agents[i].numEggBatches;
+		}
+		const uint64_t cycles = ((uint64_t)sc->horizon + 2u) / 3u;
+		if (hbc0 + cycles > 0xffffu || nb0 + cycles > 0x7fffu)
+			return fail(SAMPO_E_ARG, "horizon too long: humanBloodmealCount or numEggBatches could exceed its "
+			                         "16/15-bit field%s%ld");
+	}
 	const uint64_t P64 = initial_slots(sc, sc->mode == SAMPO_MODE_REFERENCE ? (uint64_t)sc->n_eggs : stored);
 	const uint32_t P = (uint32_t)P64;
 	h.P = P;
@@ -1814,13 +1832,22 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
+	/* Every allocation first: a failure here leaves nothing behind.  The
+	 * instance record is uploaded as an empty slot and turned into a running
+	 * one only after its inputs and initial population are in place, so on
+	 * any later failure the device record is reset to empty before the
+	 * memory it points into is freed (no kernel reads an empty slot's
+	 * pointers). */
 	Slot S;
 	S.horizon = sc->horizon;
 	S.P = P;
+	sampo_agent *d_in = nullptr; /* staging of the initial population for k_import */
 	hipError_t err = hipMalloc(&S.mem, L.total);
 	if (err == hipSuccess) err = hipMalloc(&S.agents, (size_t)2 * NFIELD * P * sizeof(uint32_t));
+	if (err == hipSuccess && stored) err = hipMalloc(&d_in, (size_t)stored * sizeof(sampo_agent));
 	if (err != hipSuccess) {
 		(void)hipGetLastError();
+		if (d_in) (void)hipFree(d_in);
 		free_slot(S);
 		return fail(SAMPO_E_NOMEM, "device allocation failed: %s%ld", hipGetErrorString(err));
 	}
@@ -1856,17 +1883,22 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		r.open = 0u;
 		for (int q = 0; q < 8; ++q) r.open |= (uint32_t)sm_open(&sp, q, hr) << q;
 	}
+	const uint32_t run_status = h.status, run_begin = h.needs_begin;
+	h.status = SAMPO_EMPTY; /* inert until the population is in place */
+	h.needs_begin = 0u;
+	bool uploaded = false;
 	err = hipMemcpy((void *)h.seeds, seeds.data(), seeds_n * sizeof(uint32_t), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy((void *)h.temp, sc->temp, (size_t)sc->horizon * sizeof(float), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemcpy((void *)h.env, sc->env, (size_t)sc->env_count * sizeof(sampo_env), hipMemcpyHostToDevice);
 	if (err == hipSuccess) err = hipMemset(h.rows, 0, (size_t)(sc->horizon + 1u) * sizeof(sampo_row));
 	if (err == hipSuccess) err = hipMemcpy((void *)h.sprow, rows.data(), rows.size() * sizeof(SpRow), hipMemcpyHostToDevice);
 	if (err == hipSuccess && !segs.empty()) err = hipMemcpy(h.seg[0], segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice);
-	if (err == hipSuccess) err = hipMemcpy(e->d_inst + id, &h, sizeof(Inst), hipMemcpyHostToDevice);
-	if (err == hipSuccess && stored) { /* host records -> SoA, on the device */
-		sampo_agent *d_in = nullptr;
-		err = hipMalloc(&d_in, (size_t)stored * sizeof(sampo_agent));
-		if (err == hipSuccess) err = hipMemcpy(d_in, sc->init_agents, (size_t)stored * sizeof(sampo_agent), hipMemcpyHostToDevice);
+	if (err == hipSuccess) {
+		err = hipMemcpy(e->d_inst + id, &h, sizeof(Inst), hipMemcpyHostToDevice);
+		uploaded = err == hipSuccess;
+	}
+	if (err == hipSuccess && stored) { /* host records -> SoA, and the start's calcStats sums, on the device */
+		err = hipMemcpy(d_in, sc->init_agents, (size_t)stored * sizeof(sampo_agent), hipMemcpyHostToDevice);
 		if (err == hipSuccess) {
 			const uint32_t k_l0 = s0.cnt[0], k_l1 = k_l0 + s0.cnt[1];
 			const uint32_t k_a0 = k_l1 + s0.cnt[2];
@@ -1875,20 +1907,34 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 			err = hipGetLastError();
 			if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
 		}
-		if (d_in) (void)hipFree(d_in);
 	}
-	if (err != hipSuccess) {
-		free_slot(S);
-		return fail(SAMPO_E_HIP, "upload failed: %s%ld", hipGetErrorString(err));
+	if (d_in) (void)hipFree(d_in);
+	if (err == hipSuccess) { /* now it may run */
+		const uint32_t flags[2] = {run_status, run_begin};
+		static_assert(offsetof(Inst, needs_begin) == offsetof(Inst, status) + 3 * sizeof(uint32_t),
+		              "status, step, pending, needs_begin are adjacent");
+		err = hipMemcpy(&e->d_inst[id].status, &flags[0], sizeof(uint32_t), hipMemcpyHostToDevice);
+		if (err == hipSuccess) err = hipMemcpy(&e->d_inst[id].needs_begin, &flags[1], sizeof(uint32_t), hipMemcpyHostToDevice);
 	}
-	if (id + 1u > e->nslots) {
+	if (err == hipSuccess && id + 1u > e->nslots) {
 		e->nslots = id + 1u;
 		drop_graph(e);
 	}
-	/* begin iteration 0 on the device (calcStats of src/abms.c:503) */
-	enqueue_prep(e);
-	HIPCHK(hipGetLastError());
-	HIPCHK(hipStreamSynchronize(e->stream));
+	if (err == hipSuccess) { /* begin iteration 0 on the device (calcStats of src/abms.c:503) */
+		enqueue_prep(e);
+		err = hipGetLastError();
+		if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+	}
+	if (err != hipSuccess) {
+		(void)hipGetLastError();
+		if (uploaded) { /* the slot must not be run on freed memory */
+			const uint32_t empty[4] = {SAMPO_EMPTY, 0u, 0u, 0u};
+			(void)hipStreamSynchronize(e->stream);
+			(void)hipMemcpy(&e->d_inst[id].status, empty, sizeof(empty), hipMemcpyHostToDevice);
+		}
+		free_slot(S);
+		return fail(SAMPO_E_HIP, "admission failed: %s%ld", hipGetErrorString(err));
+	}
 	S.used = true;
 	e->slots[id] = S;
 	*slot_out = id;

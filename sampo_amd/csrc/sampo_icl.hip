@@ -243,13 +243,10 @@ extern "C" void icl_init_devices(cl_device_type device_type)
 	for (int i = 0; i < n; ++i) {
 		hipDeviceProp_t p;
 		HIP_OR_DIE(hipGetDeviceProperties(&p, i));
-		HIP_OR_DIE(hipSetDevice(i));
 		icl_device d;
 		memset(&d, 0, sizeof(d));
 		d.hip_device = i;
-		hipStream_t s;
-		HIP_OR_DIE(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-		d.stream = s;
+		d.stream = nullptr; /* created by icl_get_device: only the devices a driver uses are touched */
 		d.mem_size = p.totalGlobalMem;
 		d.mem_available = p.totalGlobalMem;
 		d.max_buffer_size = p.totalGlobalMem;
@@ -279,16 +276,24 @@ extern "C" cl_uint icl_get_num_devices(void) { return (cl_uint)g_devices.size();
 extern "C" icl_device *icl_get_device(cl_uint id)
 {
 	if (id >= g_devices.size()) die("icl_get_device", "no such device");
-	HIP_OR_DIE(hipSetDevice(g_devices[id].hip_device));
-	return &g_devices[id];
+	icl_device &d = g_devices[id];
+	HIP_OR_DIE(hipSetDevice(d.hip_device));
+	if (!d.stream) {
+		hipStream_t s;
+		HIP_OR_DIE(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+		d.stream = s;
+	}
+	return &d;
 }
 
 extern "C" void icl_release_devices(void)
 {
 	for (auto &d : g_devices) {
-		(void)hipSetDevice(d.hip_device);
-		(void)hipStreamSynchronize(stream_of(&d));
-		(void)hipStreamDestroy(stream_of(&d));
+		if (d.stream) {
+			(void)hipSetDevice(d.hip_device);
+			(void)hipStreamSynchronize(stream_of(&d));
+			(void)hipStreamDestroy(stream_of(&d));
+		}
 		free(d.name);
 		free(d.vendor);
 		free(d.version);

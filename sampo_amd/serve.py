@@ -35,6 +35,8 @@ from sampo_amd import workloads as W
 from sampo_amd.batch import NO_CHUNK_CAP, Tuning
 
 FINISHED = (S.DONE, S.CAPACITY, S.EXTINCT, S.PHYS_FULL)
+REJECTED = -2      # result status: could not be admitted
+GROWTH_FAILED = -3  # result status: its buffers could not grow for max_deferred days
 
 
 @dataclass
@@ -62,7 +64,7 @@ def rows_digest(rows):
 
 class Server:
     def __init__(self, max_slots=1024, reserve_bytes=4 << 30, headroom=1.25, phys_factor=1.25, tuning=None,
-                 keep_rows=False, threads=16, log=None, policy="srpt", min_active=64_000_000):
+                 keep_rows=False, threads=16, log=None, policy="srpt", min_active=64_000_000, max_deferred=16):
         if policy not in ("srpt", "all"):
             raise ValueError(f"unknown policy {policy!r}")
         self.policy = policy
@@ -75,6 +77,7 @@ class Server:
         self.keep_rows = keep_rows
         self.threads = threads
         self.log = log
+        self.max_deferred = max_deferred  # days an instance may wait for memory to grow
 
     def run(self, specs, clock="wall"):
         """Serve `specs` (arrival times in seconds, or in rounds with
@@ -84,6 +87,7 @@ class Server:
         queue = []          # (index, future) in arrival order
         running = {}        # slot -> index
         fresh = set()       # slots admitted since the last poll
+        deferred = {}       # slot -> consecutive days paused for a growth that failed
         last = None         # (status, step, agents) of that poll
         nxt = 0
         rounds = 0
@@ -115,14 +119,23 @@ class Server:
                     if need + self.reserve > free:
                         if not running:  # cannot fit even with the device to itself: reject
                             queue.pop(0)
-                            results[i].status = -2
+                            results[i].status = REJECTED
                             results[i].done = now()
                             if self.log:
                                 self.log(f"serve: request {i} needs {need / 2**30:.1f} GiB, rejected")
                             continue
                         break
                     e.set_chunk(self.tuning.chunk_for(specs[i].n0), NO_CHUNK_CAP)
-                    slot = e.admit(**kw)
+                    try:
+                        slot = e.admit(**kw)
+                    except S.SampoError as err:  # the device could not take it after all
+                        if running:
+                            break  # retry when something finishes
+                        queue.pop(0)
+                        results[i].status, results[i].done = REJECTED, now()
+                        if self.log:
+                            self.log(f"serve: request {i} rejected ({err})")
+                        continue
                     free -= need
                     queue.pop(0)
                     running[slot] = i
@@ -152,6 +165,15 @@ class Server:
                 st, step, na = e.poll(max(running) + 1)  # slots past the last running one are free
                 last, fresh = (st, step, na), set()
                 t = now()
+                for slot in list(running):  # a growth that keeps failing ends the request
+                    deferred[slot] = deferred.get(slot, 0) + 1 if st[slot] == S.GROW else 0
+                    if deferred[slot] > self.max_deferred:
+                        i = running.pop(slot)
+                        del deferred[slot]
+                        results[i].done, results[i].status, results[i].steps = t, GROWTH_FAILED, int(step[slot])
+                        e.release(slot)
+                        if self.log:
+                            self.log(f"serve: request {i} could not grow for {self.max_deferred} days, ended")
                 for slot in [s for s in running if st[s] in FINISHED]:
                     i = running.pop(slot)
                     r = results[i]
@@ -162,6 +184,7 @@ class Server:
                     if self.keep_rows:
                         r.rows = rows
                     total_steps += r.agent_steps
+                    deferred.pop(slot, None)
                     e.release(slot)
                 if self.log and days_stepped % 20 == 0:
                     self.log(f"serve: t={t:.1f} running {len(running)} queued {len(queue)} "
@@ -169,7 +192,9 @@ class Server:
         wall = time.perf_counter() - t_start
         lat = np.array([r.latency for r in results if r.done >= 0 and r.status >= 0])
         summary = {"requests": len(specs), "completed": int(len(lat)),
-                   "rejected": sum(1 for r in results if r.status == -2), "wall_s": wall, "days_stepped": days_stepped,
+                   "rejected": sum(1 for r in results if r.status == REJECTED),
+                   "growth_failed": sum(1 for r in results if r.status == GROWTH_FAILED),
+                   "wall_s": wall, "days_stepped": days_stepped,
                    "agent_steps": total_steps, "agent_steps_per_s": total_steps / wall if wall else 0.0}
         if len(lat):
             summary.update(p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),

@@ -33,6 +33,30 @@ import numpy as np
 
 DMR = (0.05, 0.03, 0.02, 0.1)  # daily mortality egg, larva, pupa, adult
 
+# The single-run workloads: one scenario with the parameters above, an
+# initial population drawn by sampo_gen_population and the temperature
+# series of seed 1.
+SINGLE_RUN = {
+    # one large run: the fused step's bandwidth at scale
+    "large": dict(
+        desc="plain single run, large: 32M initial agents, buffer capacity 128M, 365-day horizon, "
+             "config-mode mortality 0.05/0.03/0.02/0.1 per day, T(d)=25+5sin(2pi d/365)+N(0,1.5)",
+        n0=32_000_000, max_agents=128_000_000, cc_per_agent=0.025, seed=1, days=365),
+    # one small run, the size a CPU runs in minutes
+    "small": dict(
+        desc="plain single run, small: 100k initial agents, 365-day horizon, config-mode mortality",
+        n0=100_000, max_agents=16_000_000, cc_per_agent=0.025, seed=1, days=365),
+}
+
+
+def single_run_inputs(w):
+    """(hourly temperature, environment, layout, agents) of a SINGLE_RUN entry."""
+    import sampo_amd as S
+    temp = S.gen_temperature(w["seed"], w["days"])
+    layout, agents = S.gen_population(w["seed"], w["n0"], float(temp[0]))
+    env = S.make_env(cc=w["cc_per_agent"] * w["n0"])
+    return temp, env, layout, agents
+
 
 @dataclass
 class Spec:
