@@ -100,14 +100,19 @@ def test_serving_window_each_request_same_alone_and_in_the_mix():
           f"{summary['agent_steps_per_s']:.3e} agent-steps/s", flush=True)
     assert summary["completed"] == len(specs)
     for sp, r in zip(specs, results):
-        assert r.status == S.DONE and r.steps == sp.horizon
+        # a run ends at its horizon, or earlier where the reference stops: more
+        # than 0.95 x capacity agents or none (src/abms.c:305-308)
+        assert (r.status == S.DONE and r.steps == sp.horizon) or \
+            (r.status in (S.CAPACITY, S.EXTINCT) and r.steps < sp.horizon), (sp.index, r.status, r.steps)
     order = sorted(range(len(specs)), key=lambda i: specs[i].n0)
-    pick = sorted(set([order[0], order[-1], max(range(len(specs)), key=lambda i: specs[i].days)] +
+    stopped = [i for i, r in enumerate(results) if r.status != S.DONE]
+    pick = sorted(set([order[0], order[-1], max(range(len(specs)), key=lambda i: specs[i].days)] + stopped[:2] +
                       list(range(0, len(specs), max(1, len(specs) // 3)))))
     for i in pick:
         print(f"serving: request {i} (n0 {specs[i].n0}, {specs[i].days} days) alone", flush=True)
         alone = V.run_alone(specs[i])
         assert np.array_equal(alone, results[i].rows), f"request {i} differs from its run alone"
     i = order[0]
-    print(f"serving: request {i} against the oracle", flush=True)
-    assert same_rows(results[i].rows, oracle_rows(specs[i], specs[i].horizon), specs[i].horizon)
+    n = results[i].steps
+    print(f"serving: request {i} against the oracle, {n} iterations", flush=True)
+    assert same_rows(results[i].rows, oracle_rows(specs[i], n), n)
