@@ -40,6 +40,19 @@ uint64_t sampo_compact_workspace_bytes(uint64_t n);
 int sampo_compact_index(const uint8_t *keys, uint64_t n, uint32_t *dest, uint32_t *counts, void *workspace,
                         uint64_t workspace_bytes, void *stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  sampo_compact_index without the workspace argument:
+ * the workspace is a fixed array the library keeps on each GPU it runs on
+ * (525 KB, held until the process ends), so the call allocates nothing.
+ * Runs on the current device (the stream must belong to it); n at most
+ * SAMPO_COMPACT_NOALLOC_MAX_KEYS (SAMPO_E_ARG beyond).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race, so order them
+ * on one stream.  The workspace needs no clearing (each call writes all of
+ * it that it reads). */
+#define SAMPO_COMPACT_NOALLOC_MAX_KEYS (1ull << 28)
+int sampo_compact_index_noalloc(const uint8_t *keys, uint64_t n, uint32_t *dest, uint32_t *counts, void *stream);
+
 /* The same on host buffers (allocates device memory, synchronises). */
 int sampo_compact_index_host(int device, const uint8_t *keys, uint64_t n, uint32_t *dest, uint32_t counts[8]);
 

@@ -34,6 +34,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "../../include/sampo_engine.h"
 #include "../../include/sampo_compact.h"
 #include "sampo_wave.h"
@@ -438,6 +440,49 @@ extern "C" int sampo_compact_index(const uint8_t *keys, uint64_t n, uint32_t *de
 	                   (const uint32_t *)bs, vec_in, vec_out);
 	CHK(hipGetLastError());
 	return 0;
+}
+
+/* The workspace of sampo_compact_index_noalloc: per-tile counts and per-block
+ * sums for SAMPO_COMPACT_NOALLOC_MAX_KEYS keys, one copy on each GPU. */
+#define NOALLOC_TILES (SAMPO_COMPACT_NOALLOC_MAX_KEYS / CTILE)
+#define NOALLOC_WORDS ((NOALLOC_TILES + (NOALLOC_TILES + CBLK - 1u) / CBLK) * CK)
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+static_assert(NOALLOC_WORDS * 4ull == (NOALLOC_TILES + 16u) * 32u, "525 KB");
+
+namespace {
+constexpr int kMaxDevices = 64;
+/* the device address of `scratch` on each device, looked up once per device
+ * with that device current (the host-side name is not a device address) */
+void *g_scratch[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+std::once_flag g_scratch_once[kMaxDevices];
+
+int scratch_on_current_device(void **out)
+{
+	int dev = -1;
+	CHK(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxDevices) return cfail(SAMPO_E_ARG, "device index beyond the scratch table");
+	std::call_once(g_scratch_once[dev], [dev] {
+		void *p = nullptr;
+		g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		g_scratch[dev] = g_scratch_err[dev] == hipSuccess ? p : nullptr;
+	});
+	if (g_scratch_err[dev] != hipSuccess || !g_scratch[dev])
+		return cfail(SAMPO_E_HIP, "hipGetSymbolAddress(scratch)", g_scratch_err[dev]);
+	*out = g_scratch[dev];
+	return 0;
+}
+} // namespace
+
+extern "C" int sampo_compact_index_noalloc(const uint8_t *keys, uint64_t n, uint32_t *dest, uint32_t *counts,
+                                           void *stream)
+{
+	if (n > SAMPO_COMPACT_NOALLOC_MAX_KEYS)
+		return cfail(SAMPO_E_ARG, "n exceeds SAMPO_COMPACT_NOALLOC_MAX_KEYS (2^28) for the fixed workspace");
+	void *ws = nullptr;
+	const int rc = scratch_on_current_device(&ws);
+	if (rc) return rc;
+	return sampo_compact_index(keys, n, dest, counts, ws, (uint64_t)NOALLOC_WORDS * sizeof(uint32_t), stream);
 }
 
 extern "C" int sampo_compact_index_host(int device, const uint8_t *keys, uint64_t n, uint32_t *dest, uint32_t counts[8])
