@@ -244,8 +244,32 @@ struct Ctx {
 
 struct Ag {
 	float age, his, dly, csd;
-	uint32_t aux, avail, cyc, misc;
+	uint32_t aux, avail, cyc, misc; /* misc: humanBloodmealCount (0-15), numEggBatches (16-30), isFemale (31) */
 };
+
+/* the stored words of an agent (sampo_model.h) and back */
+__host__ __device__ static inline void pack_agent(const Ag &a, uint32_t (&w)[NFIELD])
+{
+	w[F_AH] = (uint32_t)a.age | ((uint32_t)a.his << 16);
+	w[F_DLY] = sc_bits_f(a.dly);
+	w[F_AUX] = a.aux;
+	w[F_CSD] = sc_bits_f(a.csd);
+	w[F_AVAIL] = a.avail;
+	w[F_CH] = (a.cyc & 0xffffu) | (a.misc << 16);
+	w[F_NF] = ((a.misc >> 16) & 0x7fffu) | ((a.misc >> 31) << 15);
+}
+
+__host__ __device__ static inline void unpack_agent(const uint32_t (&w)[NFIELD], Ag &a)
+{
+	a.age = (float)(w[F_AH] & 0xffffu);
+	a.his = (float)(w[F_AH] >> 16);
+	a.dly = sc_float(w[F_DLY]);
+	a.aux = w[F_AUX];
+	a.csd = sc_float(w[F_CSD]);
+	a.avail = w[F_AVAIL];
+	a.cyc = w[F_CH] & 0xffffu;
+	a.misc = (w[F_CH] >> 16) | ((w[F_NF] & 0x7fffu) << 16) | ((w[F_NF] >> 15) << 31);
+}
 
 /* the event sums of agent_step, one LDS word each per wave: eggs laid,
  * bites, infective bites, completed cycles, their summed lengths */
@@ -421,14 +445,10 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 
 __device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uint32_t p)
 {
-	a.age = sc_float(b[p]);
-	a.his = sc_float(b[P + p]);
-	a.dly = sc_float(b[2 * P + p]);
-	a.aux = b[3 * P + p];
-	a.csd = sc_float(b[4 * P + p]);
-	a.avail = b[5 * P + p];
-	a.cyc = b[6 * P + p];
-	a.misc = b[7 * P + p];
+	uint32_t w[NFIELD];
+#pragma unroll
+	for (int f = 0; f < NFIELD; ++f) w[f] = b[(size_t)f * P + p];
+	unpack_agent(w, a);
 }
 
 /* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3);
@@ -1246,14 +1266,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					if (me[jx].born) {
 						newborn(a[jx], T.ip.nbseed, T.ip.nb_inc, T.ip.hatch_thr, T.ip.hatch_m, T.ip.hatch_b, me[jx].g);
 					} else {
-						a[jx].age = sc_float(cur[F_AGE][o]);
-						a[jx].his = sc_float(cur[F_HIS][o]);
-						a[jx].dly = sc_float(cur[F_DLY][o]);
-						a[jx].aux = cur[F_AUX][o];
-						a[jx].csd = sc_float(cur[F_CSD][o]);
-						a[jx].avail = cur[F_AVAIL][o];
-						a[jx].cyc = cur[F_CYC][o];
-						a[jx].misc = cur[F_MISC][o];
+						uint32_t wv[NFIELD];
+#pragma unroll
+						for (int f = 0; f < NFIELD; ++f) wv[f] = cur[f][o];
+						unpack_agent(wv, a[jx]);
 					}
 				}
 			}
@@ -1283,14 +1299,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t ns2 = agent_step(x, s, me[jx].g, T, ev);
 					if (ns2 < S_DEAD) {
 						n2 = 2u * s + (ns2 != s ? 1u : 0u);
-						cur[F_AGE][o] = sc_bits_f(x.age);
-						cur[F_HIS][o] = sc_bits_f(x.his);
-						cur[F_DLY][o] = sc_bits_f(x.dly);
-						cur[F_AUX][o] = x.aux;
-						cur[F_CSD][o] = sc_bits_f(x.csd);
-						cur[F_AVAIL][o] = x.avail;
-						cur[F_CYC][o] = x.cyc;
-						cur[F_MISC][o] = x.misc;
+						uint32_t wv[NFIELD];
+						pack_agent(x, wv);
+#pragma unroll
+						for (int f = 0; f < NFIELD; ++f) cur[f][o] = wv[f];
 						if (ns2 == S_LARVA) l1de += sc_f2u(x.age / 24.0f);
 						if (ns2 >= S_IMM && (x.misc >> 31)) {
 							fem += 1u;
@@ -1411,7 +1423,7 @@ __global__ void __launch_bounds__(256) k_histogram(const uint32_t *buf, const Se
 	__syncthreads();
 	for (uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
 		const Seg sg = seg[seg_find_global(seg, nseg, k)];
-		float age = sc_float(buf[seg_slot(sg.pbase, sg.state, k - sg.k0)]);
+		float age = (float)(buf[seg_slot(sg.pbase, sg.state, k - sg.k0)] & 0xffffu); /* F_AH */
 		uint32_t b = sc_f2u(age / 24.0f);
 		atomicAdd(&bins[b < 99u ? b : 99u], 1u);
 	}
@@ -1432,16 +1444,20 @@ __global__ void __launch_bounds__(256) k_import(const sampo_agent *in, uint32_t 
 	uint32_t l1de = 0, fem = 0, pinf = 0;
 	if (k < n) {
 		const sampo_agent a = in[k];
-		const size_t Pz = P;
-		buf[k] = sc_bits_f(a.ageInHours);
-		buf[Pz + k] = sc_bits_f(a.hoursInState);
-		buf[2 * Pz + k] = sc_bits_f(a.delay);
-		buf[3 * Pz + k] = a.aux;
-		buf[4 * Pz + k] = sc_bits_f(a.csd);
-		buf[5 * Pz + k] = a.availableEggs;
-		buf[6 * Pz + k] = a.cycleLength;
-		buf[7 * Pz + k] = (a.humanBloodmealCount & 0xffffu) | ((a.numEggBatches & 0x7fffu) << 16) |
-		                  (a.isFemale ? 0x80000000u : 0u);
+		Ag g;
+		g.age = a.ageInHours;
+		g.his = a.hoursInState;
+		g.dly = a.delay;
+		g.aux = a.aux;
+		g.csd = a.csd;
+		g.avail = a.availableEggs;
+		g.cyc = a.cycleLength;
+		g.misc = (a.humanBloodmealCount & 0xffffu) | ((a.numEggBatches & 0x7fffu) << 16) |
+		         (a.isFemale ? 0x80000000u : 0u);
+		uint32_t w[NFIELD];
+		pack_agent(g, w);
+#pragma unroll
+		for (int f = 0; f < NFIELD; ++f) buf[(size_t)f * P + k] = w[f];
 		if (k >= k_larva0 && k < k_larva1) l1de = sc_f2u(a.ageInHours / 24.0f);
 		if (k >= k_adult0 && a.isFemale) {
 			fem = 1u;
@@ -1808,14 +1824,30 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	     so neither can pass start + ceil(horizon / 3).  Refuse a run in which
 	     they could wrap. */
 		uint64_t hbc0 = 0, nb0 = 0;
+		float hours0 = 0.0f;
+		uint64_t cyc0 = 0;
 		for (uint64_t i = 0; i < stored; ++i) {
-			if (sc->init_agents[i].humanBloodmealCount > hbc0) hbc0 = sc->init_agents[i].humanBloodmealCount;
-			if (sc->init_agents[i].numEggBatches > nb0) nb0 = sc->init_agents[i].numEggBatches;
+			const sampo_agent &a = sc->init_agents[i];
+			if (a.humanBloodmealCount > hbc0) hbc0 = a.humanBloodmealCount;
+			if (a.numEggBatches > nb0) nb0 = a.numEggBatches;
+			if (a.cycleLength > cyc0) cyc0 = a.cycleLength;
+			/* the hour counts are whole, non-negative numbers of hours */
+			if (!(a.ageInHours >= 0.0f && a.hoursInState >= 0.0f && a.ageInHours < 65536.0f &&
+			      a.hoursInState < 65536.0f && a.ageInHours == __builtin_floorf(a.ageInHours) &&
+			      a.hoursInState == __builtin_floorf(a.hoursInState)))
+				return fail(SAMPO_E_ARG, "agent %s%ld: ageInHours and hoursInState must be whole hours in [0, 2^16)",
+				            "", (long)i);
+			if (a.ageInHours > hours0) hours0 = a.ageInHours;
+			if (a.hoursInState > hours0) hours0 = a.hoursInState;
 		}
 		const uint64_t cycles = ((uint64_t)sc->horizon + 2u) / 3u;
 		if (hbc0 + cycles > 0xffffu || nb0 + cycles > 0x7fffu)
 			return fail(SAMPO_E_ARG, "horizon too long: humanBloodmealCount or numEggBatches could exceed its "
 			                         "16/15-bit field%s%ld");
+		/* both hour counts and cycleLength grow by at most one an iteration */
+		if ((uint64_t)hours0 + sc->horizon > 0xffffu || cyc0 + sc->horizon > 0xffffu)
+			return fail(SAMPO_E_ARG, "horizon too long: ageInHours, hoursInState or cycleLength could exceed "
+			                         "its 16-bit field%s%ld");
 	}
 	const uint64_t P64 = initial_slots(sc, sc->mode == SAMPO_MODE_REFERENCE ? (uint64_t)sc->n_eggs : stored);
 	const uint32_t P = (uint32_t)P64;

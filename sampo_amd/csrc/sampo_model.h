@@ -2,14 +2,21 @@
  * sampo_model.h -- the agent model as the fused engine evaluates it, shared
  * by the device kernels and the host-side preparation of iteration 0.
  *
- * Storage is structure-of-arrays, 8 dwords per agent (32 bytes):
- *   AGE  ageInHours            (f32)     AUX   ovipositionAttemps | cumulativeLarvalDelay bits
- *   HIS  hoursInState          (f32)     AVAIL availableEggs      (u32)
- *   DLY  delay                 (f32)     CYC   cycleLength        (u32)
- *   CSD  cumulativeSporogonicDevelopment MISC  humanBloodmealCount (bits 0-15),
- *                                              numEggBatches (16-30), isFemale (31)
- * The agent's state is implied by the range it is stored in and `dead` lives
- * only in registers, so neither is stored.
+ * Storage is structure-of-arrays, 7 dwords per agent (28 bytes):
+ *   AH   ageInHours (bits 0-15) and hoursInState (16-31) as integers
+ *   DLY  delay                             (f32)
+ *   AUX  ovipositionAttemps | cumulativeLarvalDelay bits
+ *   CSD  cumulativeSporogonicDevelopment    (f32)
+ *   AVAIL availableEggs                    (u32)
+ *   CH   cycleLength (bits 0-15), humanBloodmealCount (16-31)
+ *   NF   numEggBatches (bits 0-14), isFemale (15)
+ * The two hour counts are whole numbers of hours: they start at 0 or 24
+ * and grow by the step of one hour (src/abms.c:15), so each is held exactly
+ * in 16 bits while it stays below 2^16; cycleLength grows by at most one an
+ * iteration.  Admission refuses initial populations and horizons that could
+ * take any of them past their field (sampo_engine_admit).  The agent's state
+ * is implied by the range it is stored in and `dead` lives only in
+ * registers, so neither is stored.
  *
  * Index spaces of one instance at the start of an iteration:
  *   dense k      0..n-1, the states egg..gravid concatenated without gaps;
@@ -30,7 +37,7 @@
 #include "sampo_canon.h"
 #include "../../include/sampo_engine.h"
 
-enum { F_AGE = 0, F_HIS, F_DLY, F_AUX, F_CSD, F_AVAIL, F_CYC, F_MISC, NFIELD };
+enum { F_AH = 0, F_DLY, F_AUX, F_CSD, F_AVAIL, F_CH, F_NF, NFIELD };
 enum { S_EGG = 0, S_LARVA, S_PUPA, S_IMM, S_MS, S_BMS, S_BMD, S_GRAVID, S_DEAD };
 
 #define SAMPO_PAD 64u  /* LOCAL_SIZE of the reference layout (include/agent.h:16) */

@@ -591,7 +591,7 @@ for _f, _e in zip(CASES[-len(NEWBORN_EDGES):], NEWBORN_EDGES):
 def histogram_population():
     """Adults with ages at the bin edges of ageHistogram.cl:30 (days = hours
     / 24 in binary32, capped at bin 99), plus non-adults that must not count."""
-    ages = [0, 23, 24, 25, 47, 48, 1000, 2376, 2399, 2400, 5000, 100000]
+    ages = [0, 23, 24, 25, 47, 48, 1000, 2376, 2399, 2400, 5000, 60000]
     return {EGG: [dict(delay=1000.0)], LARVA: [dict(ageInHours=500, delay=1e9)],
             IMM: [dict(ageInHours=a, delay=1e9) for a in ages[:4]],
             MS: [dict(ageInHours=a, delay=1e9, isFemale=0) for a in ages[4:8]],
