@@ -228,6 +228,7 @@ struct Inst {
 	uint32_t P_max;        /* physical slots the buffers may grow to */
 	uint32_t chunk_size, target_chunks, cmax, smax;
 	uint32_t status, step, pending, needs_begin;
+	uint32_t compact;      /* immature states store only their non-zero words (sm_fields) */
 	uint32_t l1de, females, potinf, ncyc, scyc, bites, ibites; /* stats of the start of `step` */
 	float hmr_stage[4];
 	float hmr_adult[100];
@@ -259,8 +260,11 @@ __host__ __device__ static inline void pack_agent(const Ag &a, uint32_t (&w)[NFI
 	w[F_NF] = ((a.misc >> 16) & 0x7fffu) | ((a.misc >> 31) << 15);
 }
 
-__host__ __device__ static inline void unpack_agent(const uint32_t (&w)[NFIELD], Ag &a)
+__host__ __device__ static inline void unpack_agent(const uint32_t (&w0)[NFIELD], Ag &a, uint32_t fields = SM_ALL_FIELDS)
 {
+	uint32_t w[NFIELD];
+#pragma unroll
+	for (int f = 0; f < NFIELD; ++f) w[f] = (fields >> f) & 1u ? w0[f] : 0u; /* words not stored are zero */
 	a.age = (float)(w[F_AH] & 0xffffu);
 	a.his = (float)(w[F_AH] >> 16);
 	a.dly = sc_float(w[F_DLY]);
@@ -285,7 +289,7 @@ struct TileParams {
 	const Chunk *chk;     /* the instance's chunks of this iteration */
 	uint32_t *cout;
 	Chunk ch;
-	uint32_t P, nseg, cmax;
+	uint32_t P, nseg, cmax, compact;
 };
 
 __device__ static inline float rnd(const uint32_t *z, uint32_t g)
@@ -443,12 +447,12 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	}
 }
 
-__device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uint32_t p)
+__device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uint32_t p, uint32_t fields)
 {
 	uint32_t w[NFIELD];
 #pragma unroll
-	for (int f = 0; f < NFIELD; ++f) w[f] = b[(size_t)f * P + p];
-	unpack_agent(w, a);
+	for (int f = 0; f < NFIELD; ++f) w[f] = (fields >> f) & 1u ? b[(size_t)f * P + p] : 0u;
+	unpack_agent(w, a, fields);
 }
 
 /* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3);
@@ -1131,10 +1135,12 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				s_sg[o] = s_vb[qq] + r;
 				if (!born) {
 					const uint32_t p = seg_slot(pb, st, r);
+					const uint32_t fm = sm_fields(st & 0xffu, T.compact);
 #pragma unroll
 					for (int f = 0; f < NFIELD; ++f)
-						__builtin_amdgcn_global_load_lds((glob_void *)at_slot(fb[f], p),
-						                                 (lds_void *)&buf[f][(uint32_t)jx * WG + w * 64u], 4, 0, 0);
+						if ((fm >> f) & 1u)
+							__builtin_amdgcn_global_load_lds((glob_void *)at_slot(fb[f], p),
+							                                 (lds_void *)&buf[f][(uint32_t)jx * WG + w * 64u], 4, 0, 0);
 				}
 			}
 		}
@@ -1204,6 +1210,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				T.cout = I->cout;
 				T.cmax = I->cmax;
 				T.P = I->P;
+				T.compact = I->compact;
 			}
 			lds_barrier();
 		}
@@ -1269,7 +1276,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 						uint32_t wv[NFIELD];
 #pragma unroll
 						for (int f = 0; f < NFIELD; ++f) wv[f] = cur[f][o];
-						unpack_agent(wv, a[jx]);
+						unpack_agent(wv, a[jx], sm_fields(me[jx].s, T.compact));
 					}
 				}
 			}
@@ -1341,8 +1348,13 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t o = (uint32_t)jx * WG + tid;
 					const uint32_t off = s_wc[jx][w][cl] + lr[jx];
 					const uint32_t p = (cl & 1u) ? s_end[cl >> 1] - 1u - off : s_head[cl >> 1] + off;
+					/* the words the agent's new state keeps: a mover from state s is now in
+					 * s + 1 (gravid: blood-meal seeking, which keeps all) */
+					const uint32_t sn = (cl >> 1) + (cl & 1u);
+					const uint32_t fm = sm_fields(sn > S_GRAVID ? S_BMS : sn, T.compact);
 #pragma unroll
-					for (int f = 0; f < NFIELD; ++f) *at_slot(fout[f], p) = cur[f][o];
+					for (int f = 0; f < NFIELD; ++f)
+						if ((fm >> f) & 1u) *at_slot(fout[f], p) = cur[f][o];
 				}
 			}
 			/* the next tile rewrites s_wc only after its first barrier */
@@ -1392,14 +1404,14 @@ struct Hatch {
 };
 
 __global__ void k_export(const uint32_t *buf, uint32_t P, const Seg *seg, uint32_t nseg, uint32_t n,
-                         const uint32_t *nbz, Hatch hb, sampo_agent *out)
+                         const uint32_t *nbz, Hatch hb, uint32_t compact, sampo_agent *out)
 {
 	const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n) return;
 	const Seg sg = seg[seg_find_global(seg, nseg, k)];
 	Ag a;
 	if (sg.pbase == PB_NEWBORN) newborn(a, nbz, hb.inc, hb.thr, hb.m, hb.b, sg.vbase + (k - sg.k0));
-	else load_agent(a, buf, P, seg_slot(sg.pbase, sg.state, k - sg.k0));
+	else load_agent(a, buf, P, seg_slot(sg.pbase, sg.state, k - sg.k0), sm_fields(sg.state & 0xffu, compact));
 	sampo_agent r;
 	r.ageInHours = a.age;
 	r.hoursInState = a.his;
@@ -1849,6 +1861,24 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 			return fail(SAMPO_E_ARG, "horizon too long: ageInHours, hoursInState or cycleLength could exceed "
 			                         "its 16-bit field%s%ld");
 	}
+	{ /* immatures keep only their non-zero words (sm_fields) when the inputs
+	     guarantee the other words stay zero: finite temperatures, and initial
+	     eggs, larvae and pupae with zero development sum, eggs, cycle length,
+	     blood meals and batches (and eggs with no larval delay) */
+		h.compact = 1u;
+		for (uint32_t t = 0; t < sc->horizon && h.compact; ++t)
+			if (!__builtin_isfinite(sc->temp[t])) h.compact = 0u;
+		uint64_t k = 0;
+		for (uint32_t st = 0; st <= S_PUPA && stored && h.compact; ++st)
+			for (uint32_t i = 0; i < s0.cnt[st]; ++i, ++k) {
+				const sampo_agent &a = sc->init_agents[k];
+				if (sc_bits_f(a.csd) || a.availableEggs || a.cycleLength || a.humanBloodmealCount ||
+				    a.numEggBatches || (st == S_EGG && a.aux)) {
+					h.compact = 0u;
+					break;
+				}
+			}
+	}
 	const uint64_t P64 = initial_slots(sc, sc->mode == SAMPO_MODE_REFERENCE ? (uint64_t)sc->n_eggs : stored);
 	const uint32_t P = (uint32_t)P64;
 	h.P = P;
@@ -2228,7 +2258,7 @@ extern "C" int sampo_engine_export(sampo_engine *e, uint32_t slot, sampo_layout 
 	memcpy(hb.m, ip.hatch_m, sizeof hb.m);
 	memcpy(hb.b, ip.hatch_b, sizeof hb.b);
 	hipLaunchKernelGGL(k_export, dim3((ip.n + 255u) / 256u), dim3(256), 0, e->stream, h.buf[ip.cur], h.P, h.seg[par],
-	                   ip.nseg, ip.n, nbz, hb, d);
+	                   ip.nseg, ip.n, nbz, hb, h.compact, d);
 	hipError_t err = hipGetLastError();
 	if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
 	if (err == hipSuccess) err = hipMemcpy(agents, d, (size_t)ip.n * sizeof(sampo_agent), hipMemcpyDeviceToHost);

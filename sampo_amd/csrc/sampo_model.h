@@ -18,6 +18,17 @@
  * is implied by the range it is stored in and `dead` lives only in
  * registers, so neither is stored.
  *
+ * Words that are zero for every agent of a state are not stored for it
+ * (sm_fields): an egg keeps AH, DLY and NF, a larva or pupa AH, DLY, AUX and
+ * NF, an adult all seven.  This holds because the update rules
+ * (kernel/updateAgents.cl:39-270) only set cumulativeSporogonicDevelopment,
+ * availableEggs, cycleLength, humanBloodmealCount and numEggBatches in adult
+ * states (the development sum grows only once humanBloodmealCount > 0, and
+ * only by a finite amount at a finite temperature), and an egg's
+ * cumulativeLarvalDelay is the zero initAgents.cl:53 gives it.  An instance
+ * whose initial population or temperature series does not meet these
+ * conditions stores all seven words in every state (Inst.compact = 0).
+ *
  * Index spaces of one instance at the start of an iteration:
  *   dense k      0..n-1, the states egg..gravid concatenated without gaps;
  *   virtual g    the reference's padded index (ranges start at multiples of
@@ -39,6 +50,16 @@
 
 enum { F_AH = 0, F_DLY, F_AUX, F_CSD, F_AVAIL, F_CH, F_NF, NFIELD };
 enum { S_EGG = 0, S_LARVA, S_PUPA, S_IMM, S_MS, S_BMS, S_BMD, S_GRAVID, S_DEAD };
+
+#define SM_ALL_FIELDS ((1u << NFIELD) - 1u)
+
+/* the stored words of state s: a bit per word (the others are zero) */
+SC_FN uint32_t sm_fields(uint32_t s, uint32_t compact)
+{
+	if (!compact || s > S_PUPA) return SM_ALL_FIELDS;
+	const uint32_t imm = (1u << F_AH) | (1u << F_DLY) | (1u << F_NF);
+	return s == S_EGG ? imm : imm | (1u << F_AUX);
+}
 
 #define SAMPO_PAD 64u  /* LOCAL_SIZE of the reference layout (include/agent.h:16) */
 
