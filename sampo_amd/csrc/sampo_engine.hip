@@ -1765,6 +1765,8 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	if (sc->mode != SAMPO_MODE_REFERENCE && sc->mode != SAMPO_MODE_CONFIG)
 		return fail(SAMPO_E_ARG, "unknown mode%s%ld");
 	if (sc->max_agents == 0) return fail(SAMPO_E_ARG, "max_agents is required%s%ld");
+	for (uint32_t t = 0; t < sc->horizon; ++t) /* the species functions index tables by temperature */
+		if (!__builtin_isfinite(sc->temp[t])) return fail(SAMPO_E_ARG, "temperature %s%ld is not finite", "", (long)t);
 	/* slots are addressed with 32-bit byte offsets per field */
 	if (round256(sc->max_agents) > (1ull << 30)) return fail(SAMPO_E_ARG, "max_agents exceeds 2^30%s%ld");
 	uint32_t id = e->max_slots;
@@ -1861,13 +1863,12 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 			return fail(SAMPO_E_ARG, "horizon too long: ageInHours, hoursInState or cycleLength could exceed "
 			                         "its 16-bit field%s%ld");
 	}
-	{ /* immatures keep only their non-zero words (sm_fields) when the inputs
-	     guarantee the other words stay zero: finite temperatures, and initial
-	     eggs, larvae and pupae with zero development sum, eggs, cycle length,
-	     blood meals and batches (and eggs with no larval delay) */
+	{ /* immatures keep only their non-zero words (sm_fields) when the initial
+	     eggs, larvae and pupae have zero development sum, eggs, cycle length,
+	     blood meals and batches (and the eggs no larval delay): the update
+	     rules then keep those words zero (temperatures are finite, checked
+	     below) */
 		h.compact = 1u;
-		for (uint32_t t = 0; t < sc->horizon && h.compact; ++t)
-			if (!__builtin_isfinite(sc->temp[t])) h.compact = 0u;
 		uint64_t k = 0;
 		for (uint32_t st = 0; st <= S_PUPA && stored && h.compact; ++st)
 			for (uint32_t i = 0; i < s0.cnt[st]; ++i, ++k) {

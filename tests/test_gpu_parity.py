@@ -77,6 +77,52 @@ def test_engine_config_mode_matches_oracle(n0, days, seed, dmr):
         agents_equal(ag_o, ag_g)
 
 
+@pytest.mark.parametrize("field", ["csd", "humanBloodmealCount", "aux", "cycleLength"])
+def test_engine_full_record_when_immatures_carry_adult_words(field):
+    """Initial eggs, larvae and pupae that carry words an immature keeps at
+    zero (here: a development sum, blood meals, an egg's larval delay sum or
+    a cycle length) make the instance store every word in every state; the
+    run still equals the oracle, including those agents' own fields."""
+    n0, days, seed, dmr = 4000, 6, 21, (0.04, 0.03, 0.02, 0.1)
+    steps = 24 * days
+    temp, lay, ag = config_inputs(n0, days, seed)
+    rng = np.random.default_rng(3)
+    n_imm = int(lay[2][1] - lay[2][0] + lay[1][1] - lay[1][0] + lay[0][1] - lay[0][0])
+    pick = rng.choice(n_imm, 200, replace=False)
+    ag = ag.copy()
+    if field == "csd":
+        ag["csd"][pick] = rng.uniform(0.0, 2.0, 200).astype(np.float32)
+    elif field == "aux":  # only an egg's larval delay sum is zero by the rules
+        pick = pick[pick < int(lay[0][1])]
+        ag["aux"][pick] = np.float32(0.25).view(np.uint32)
+    else:
+        ag[field][pick] = rng.integers(1, 5, 200)
+    env = S.make_env(cc=5.0 * n0)
+    sim = O.Sim(temp=temp, env=env, capacity=60 * n0, seed=seed, layout=lay, agents=ag, mode=1, dmr=dmr)
+    orows = sim.run(steps)
+    with S.Engine(max_slots=1) as e:
+        slot = e.admit(mode=S.MODE_CONFIG, seed=seed, horizon=steps, temp=temp, env=env, max_agents=60 * n0,
+                       dmr=dmr, layout=lay, agents=ag)
+        l0, a0 = e.export(slot)
+        agents_equal(ag, a0)  # stored and read back unchanged before any iteration
+        e.step(steps)
+        rows_equal(orows, e.rows(slot, 0, steps))
+        lay_o, ag_o, _ = sim.export()
+        lay_g, ag_g = e.export(slot)
+        assert np.array_equal(lay_o, lay_g)
+        agents_equal(ag_o, ag_g)
+
+
+def test_engine_refuses_non_finite_temperatures():
+    temp, lay, ag = config_inputs(500, 2, 5)
+    temp = temp.copy()
+    temp[30] = np.nan
+    with S.Engine(max_slots=1) as e:
+        with pytest.raises(S.SampoError):
+            e.admit(mode=S.MODE_CONFIG, seed=5, horizon=48, temp=temp, env=S.make_env(cc=2500.0), max_agents=20000,
+                    layout=lay, agents=ag)
+
+
 def intervention_env(steps, cc, seed):
     """A time-varying environment with every intervention switched on in
     some hours: larvicide, ITN, IRS and ovitrap coverage x effectiveness
