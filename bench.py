@@ -169,7 +169,7 @@ def batched_measure(args, workload, log):
     n_total = sum(sp.n0 for sp in specs)
     log(f"{workload}: {len(specs)} scenarios, {n_total / 1e9:.3f}e9 initial agents")
     tun, tinfo = tuning_for(workload, args, specs, log)
-    log(f"tuning ({tinfo['source']}): tile {tun.tile}, chunk per bucket {tun.chunk}")
+    log(f"tuning ({tinfo['source']}): tile per bucket {tun.tiles} (else {tun.tile}), chunk per bucket {tun.chunk}")
     sample = sorted(set(np.linspace(0, len(specs) - 1, min(args.plain_sample, len(specs))).round().astype(int)))
     bres, brows = B.run_batched(specs, tun, warm_days=args.batched_warmup, timed_days=args.batched_days,
                                 keep_rows=sample, log=log)
@@ -196,6 +196,7 @@ def batched_measure(args, workload, log):
                      "frac": bres["pass_GBs"] / HBM_PEAK_GBS, "kernel": "k_pass",
                      "bytes": "34 B per agent in + 34 B per agent out of each iteration, HIP events over one day"},
         "tile": "x".join(map(str, tun.tile)), "chunk_per_bucket": {str(k): v for k, v in sorted(tun.chunk.items())},
+        "tile_per_bucket": {str(k): "x".join(map(str, v)) for k, v in sorted(tun.tiles.items())},
         "tuning": {**tinfo, **tun.to_json()},
         "detail": bres,
     }

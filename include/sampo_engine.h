@@ -162,9 +162,13 @@ const char *sampo_last_error(void);
 int sampo_engine_create(int device, uint32_t max_slots, sampo_engine **out);
 void sampo_engine_destroy(sampo_engine *e);
 
-/* Tile shape of the fused pass: work-group size, agents per thread and the
- * minimum waves per SIMD the registers are capped for (0: no cap).  Only
- * shapes compiled into the library are accepted (SAMPO_E_ARG otherwise). */
+/* Tile shape of the fused pass for the instances admitted from now on:
+ * work-group size, agents per thread and the minimum waves per SIMD the
+ * registers are capped for (0: no cap).  Instances of different shapes are
+ * advanced by one pass launch per shape each iteration, up to 4 shapes in
+ * use at once (SAMPO_E_FULL beyond; a shape no instance uses any more is
+ * replaced).  Results do not depend on the shape.  Only shapes compiled into
+ * the library are accepted (SAMPO_E_ARG otherwise). */
 int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt, uint32_t waves);
 
 /* Persistent work-groups of the pass per CU (0: as many as the tile shape

@@ -18,13 +18,17 @@ Plain paths, one scenario at a time:
 
 Autotuner: initial sizes fall in buckets of a factor of 4 from 1e3.  A
 uniform sample of the set large enough to fill the GPU (~3e8 agents) is
-admitted for each trial and timed over one day after a warm-up day.  For
-each bucket in turn (largest share of agents first) its instances' chunk
-size -- agents per work unit of the pass and per output run -- is set to
-each candidate with the other buckets' kept, and the fastest in
-agent-steps/s is kept; then the tile shape of the pass (work-group size,
-agents per thread, register cap), which is one per launch, is chosen the
-same way.  Instances of a bucket are admitted with its chunk size.
+admitted for each trial and timed over one day after a warm-up day.  Two
+rounds of coordinate descent, each visiting the buckets largest share of
+agents first: for a bucket, its instances' value is set to each candidate
+with the other buckets' kept, and the fastest in agent-steps/s is kept.
+The first round tunes the chunk size (agents per work unit of the pass and
+per output run), the second the tile shape of the pass (work-group size,
+agents per thread, register cap).  Instances of different tile shapes are
+advanced by one pass launch per shape, so a bucket's shape is its own; the
+engine runs at most 4 shapes at once, and candidates that would need a
+fifth are skipped.  Instances of a bucket are admitted with its chunk size
+and tile shape.
 """
 import json
 import math
@@ -38,7 +42,8 @@ from sampo_amd import workloads as W
 
 BUCKET_BASE = 1e3
 CHUNK_CANDIDATES = (4096, 16384, 65536, 262144)
-TILE_CANDIDATES = ((768, 1, 6), (512, 1, 6), (256, 1, 8), (1024, 1, 6))
+TILE_CANDIDATES = ((768, 1, 6), (512, 1, 6), (256, 1, 8), (1024, 1, 6), (512, 2, 0), (256, 2, 6))
+MAX_TILE_SHAPES = 4  # pass launches per iteration the engine keeps (MAX_CLASSES)
 DEFAULT_CHUNK = 16384
 NO_CHUNK_CAP = 1 << 24  # target chunks per instance: effectively unlimited
 
@@ -53,28 +58,57 @@ def bucket_range(b):
 
 @dataclass
 class Tuning:
-    tile: tuple = (768, 1, 6)
-    chunk: dict = field(default_factory=dict)  # bucket -> agents per chunk
-    table: list = field(default_factory=list)  # measurements
+    tile: tuple = (768, 1, 6)                    # shape of buckets without one of their own
+    chunk: dict = field(default_factory=dict)   # bucket -> agents per chunk
+    tiles: dict = field(default_factory=dict)   # bucket -> tile shape
+    table: list = field(default_factory=list)   # measurements
 
     def chunk_for(self, n0):
         return self.chunk.get(bucket_of(n0), DEFAULT_CHUNK)
 
+    def tile_for(self, n0):
+        return tuple(self.tiles.get(bucket_of(n0), self.tile))
+
     def to_json(self):
         return {"tile": list(self.tile), "chunk": {str(k): v for k, v in sorted(self.chunk.items())},
-                "table": self.table}
+                "tiles": {str(k): list(v) for k, v in sorted(self.tiles.items())}, "table": self.table}
 
 
 def admit_specs(engine, specs, tuning, phys_factor=0.0, kws=None):
-    """Admit specs in order with the tuned chunk size of each one's bucket;
-    returns the slots.  kws: already materialised admit kwargs (else
-    generated on a thread pool)."""
+    """Admit specs in order with the tuned chunk size and tile shape of each
+    one's bucket; returns the slots.  kws: already materialised admit kwargs
+    (else generated on a thread pool)."""
     slots = []
     src = zip(specs, kws) if kws is not None else W.materialise_all(specs, phys_factor)
     for sp, kw in src:
+        engine.set_tile(*tuning.tile_for(sp.n0))
         engine.set_chunk(tuning.chunk_for(sp.n0), NO_CHUNK_CAP)
         slots.append(engine.admit(**kw))
     return slots
+
+
+def coordinate_descent(order, candidates, state, measure, allowed=lambda state: True, log=None, what=""):
+    """One round of coordinate descent: for each key of `order` in turn, try
+    every candidate value for it with the other keys kept, and keep the
+    value with the highest measure(state).  Trials whose state fails
+    allowed() are skipped.  Returns (state, best, table of trials)."""
+    state = dict(state)
+    best = measure(state)
+    table = [{"start": True, "state": {str(k): v for k, v in state.items()}, "rate": best}]
+    for key in order:
+        for c in candidates:
+            if c == state[key]:
+                continue
+            trial = {**state, key: c}
+            if not allowed(trial):
+                continue
+            rate = measure(trial)
+            table.append({"key": key, "value": c, "rate": rate})
+            if log:
+                log(f"autotune {what} bucket {key} = {c}: {rate:.3e} agent-steps/s")
+            if rate > best:
+                best, state = rate, trial
+    return state, best, table
 
 
 def _day_rate(engine, days=1):
@@ -101,50 +135,41 @@ def tuning_mix(specs, agents=300_000_000, max_instances=1024):
 
 def _mix_rate(mix, kws, tuning):
     with S.Engine(max_slots=len(mix)) as e:
-        e.set_tile(*tuning.tile)
         admit_specs(e, mix, tuning, kws=kws)
         return _day_rate(e)
 
 
 def autotune(specs, chunks=CHUNK_CANDIDATES, tiles=TILE_CANDIDATES, mix_agents=300_000_000, log=None):
-    """Coordinate descent on a mix of the set: for each size bucket, largest
-    share of agents first, the chunk size of that bucket's instances is set
-    to each candidate while the others keep theirs, and the fastest day in
-    agent-steps/s wins; then the tile shape of the pass the same way."""
+    """Coordinate descent on a mix of the set: the chunk size per size
+    bucket, then the tile shape per size bucket (at most MAX_TILE_SHAPES
+    distinct shapes counting the default one, which buckets outside the mix
+    keep), largest share of agents first, keeping what gives the
+    most agent-steps/s over a day."""
     mix = tuning_mix(specs, mix_agents)
     kws = [kw for _, kw in W.materialise_all(mix)]
     share = {}
     for sp in mix:
         share[bucket_of(sp.n0)] = share.get(bucket_of(sp.n0), 0) + sp.n0
-    tun = Tuning(chunk={b: DEFAULT_CHUNK for b in share})
-    best_rate, _ = _mix_rate(mix, kws, tun)
-    tun.table.append({"mix": len(mix), "start": True, "agent_steps_per_s": best_rate})
+    order = sorted(share, key=lambda x: -share[x])
+    base = Tuning()
     if log:
-        log(f"autotune: mix of {len(mix)} scenarios, {sum(share.values()) / 1e6:.0f}M agents: "
-            f"{best_rate:.3e} agent-steps/s at chunk {DEFAULT_CHUNK}")
-    for b in sorted(share, key=lambda x: -share[x]):
-        for c in chunks:
-            if c == tun.chunk[b]:
-                continue
-            trial = Tuning(tile=tun.tile, chunk={**tun.chunk, b: c})
-            rate, ms = _mix_rate(mix, kws, trial)
-            tun.table.append({"bucket": b, "n0_range": bucket_range(b), "chunk": c, "agent_steps_per_s": rate,
-                              "day_ms": ms})
-            if log:
-                log(f"autotune bucket {b} chunk {c}: {rate:.3e} agent-steps/s")
-            if rate > best_rate:
-                best_rate, tun.chunk[b] = rate, c
-    for tile in tiles:
-        if tuple(tile) == tuple(tun.tile):
-            continue
-        trial = Tuning(tile=tuple(tile), chunk=dict(tun.chunk))
-        rate, ms = _mix_rate(mix, kws, trial)
-        tun.table.append({"tile": list(tile), "agent_steps_per_s": rate, "day_ms": ms})
-        if log:
-            log(f"autotune tile {tile}: {rate:.3e} agent-steps/s")
-        if rate > best_rate:
-            best_rate, tun.tile = rate, tuple(tile)
-    return tun
+        log(f"autotune: mix of {len(mix)} scenarios, {sum(share.values()) / 1e6:.0f}M agents")
+
+    def rate_of(chunk, tile):
+        return _mix_rate(mix, kws, Tuning(tile=base.tile, chunk=dict(chunk), tiles=dict(tile)))[0]
+
+    chunk, _, t1 = coordinate_descent(order, chunks, {b: DEFAULT_CHUNK for b in order},
+                                      lambda c: rate_of(c, {b: base.tile for b in order}), log=log, what="chunk")
+    tile, best, t2 = coordinate_descent(order, [tuple(t) for t in tiles], {b: tuple(base.tile) for b in order},
+                                        lambda t: rate_of(chunk, t),
+                                        allowed=lambda t: len(set(t.values()) | {tuple(base.tile)}) <= MAX_TILE_SHAPES, log=log,
+                                        what="tile")
+    table = [{"round": "chunk", **r} for r in t1] + [{"round": "tile", **{k: (list(v) if isinstance(v, tuple) else v)
+                                                                           for k, v in r.items()}} for r in t2]
+    for r in table:  # JSON keys
+        if "state" in r:
+            r["state"] = {k: (list(v) if isinstance(v, tuple) else v) for k, v in r["state"].items()}
+    return Tuning(tile=base.tile, chunk=chunk, tiles=tile, table=table)
 
 
 def run_batched(specs, tuning, warm_days=1, timed_days=7, phys_factor=1.25, headroom=1.25, keep_rows=(), log=None):
@@ -156,7 +181,6 @@ def run_batched(specs, tuning, warm_days=1, timed_days=7, phys_factor=1.25, head
     free0, total = S.device_memory()
     t_admit = time.perf_counter()
     with S.Engine(max_slots=len(specs)) as e:
-        e.set_tile(*tuning.tile)
         slots = admit_specs(e, specs, tuning, phys_factor)
         e.sync()
         admit_s = time.perf_counter() - t_admit
@@ -247,4 +271,4 @@ def save_tuning(tuning, path):
 def load_tuning(path):
     d = json.load(open(path))
     return Tuning(tile=tuple(d["tile"]), chunk={int(k): int(v) for k, v in d["chunk"].items()},
-                  table=d.get("table", []))
+                  tiles={int(k): tuple(v) for k, v in d.get("tiles", {}).items()}, table=d.get("table", []))

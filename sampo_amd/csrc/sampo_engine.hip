@@ -229,15 +229,22 @@ struct Inst {
 	uint32_t chunk_size, target_chunks, cmax, smax;
 	uint32_t status, step, pending, needs_begin;
 	uint32_t compact;      /* immature states store only their non-zero words (sm_fields) */
+	uint32_t tile_class;   /* the pass launch (tile shape) that advances it */
 	uint32_t l1de, females, potinf, ncyc, scyc, bites, ibites; /* stats of the start of `step` */
 	float hmr_stage[4];
 	float hmr_adult[100];
 	IterState st[2];       /* iteration `step` uses st[step & 1] */
 };
 
+/* Instances are admitted under one of up to MAX_CLASSES tile shapes of the
+ * pass (work-group size, agents per thread, register cap); each iteration
+ * launches one pass per shape, over the chunks of that shape's instances,
+ * with a ticket counter and a chunk total of its own. */
+#define MAX_CLASSES 4u
+
 struct Ctx {
-	uint32_t ticket;
-	uint32_t total;
+	uint32_t ticket[MAX_CLASSES];
+	uint32_t total[MAX_CLASSES];
 	unsigned long long agent_steps;
 };
 
@@ -482,77 +489,84 @@ __device__ static inline uint32_t wave_sum(uint32_t v)
 
 /* ------------------------------------------------------------ plan */
 
-/* Lay every running instance's chunks end to end for the next pass:
- * pl_base[i] = first ticket of instance i, ctx->total = all chunks; reset
- * the ticket.  Run by one work-group of 1024 threads: k_plan after the
- * prep of several slots, after a release, or k_hold.  The instances' fields
- * are read with atomic loads at agent scope. */
-__device__ static void plan_all(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst)
+/* Lay every running instance's chunks end to end for the next pass, per
+ * tile class c: pl_base[c * (ninst + 1) + i] = first ticket of instance i
+ * among class c's chunks (instances of other classes count no chunks),
+ * ctx->total[c] = class c's chunks; reset the tickets.  Run by one
+ * work-group of 1024 threads: k_plan after the prep of several slots, after
+ * a release, or k_hold.  The instances' fields are read with atomic loads at
+ * agent scope. */
+__device__ static void plan_all(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst,
+                                uint32_t ncls)
 {
 	__shared__ uint32_t s_scan[1024];
 	const uint32_t tid = threadIdx.x;
-	auto chunks_of = [&](uint32_t i) -> uint32_t {
+	auto chunks_of = [&](uint32_t i, uint32_t c) -> uint32_t {
 		Inst *I = inst + i;
 		const uint32_t status = __hip_atomic_load(&I->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		const uint32_t par = __hip_atomic_load(&I->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u;
 		const uint32_t active = __hip_atomic_load(&I->st[par].active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const uint32_t cl = __hip_atomic_load(&I->tile_class, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		pl_par[i] = (uint8_t)par;
-		return status == SAMPO_RUNNING && active
+		return status == SAMPO_RUNNING && active && cl == c
 		           ? __hip_atomic_load(&I->st[par].nchunk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
 		           : 0u;
 	};
-	if (ninst <= 64u) { /* one wave, no barriers */
-		if (tid < 64u) {
-			const uint32_t nc = tid < ninst ? chunks_of(tid) : 0u;
-			uint32_t x = nc;
-			for (int o = 1; o < 64; o <<= 1) {
-				uint32_t y = __shfl_up(x, o, 64);
-				if (tid >= (uint32_t)o) x += y;
+	for (uint32_t c = 0; c < ncls; ++c) {
+		uint32_t *pb = pl_base + c * (ninst + 1u);
+		if (ninst <= 64u) { /* one wave, no barriers */
+			if (tid < 64u) {
+				const uint32_t nc = tid < ninst ? chunks_of(tid, c) : 0u;
+				uint32_t x = nc;
+				for (int o = 1; o < 64; o <<= 1) {
+					uint32_t y = __shfl_up(x, o, 64);
+					if (tid >= (uint32_t)o) x += y;
+				}
+				if (tid < ninst) pb[tid] = x - nc;
+				const uint32_t all = __shfl(x, 63, 64);
+				if (tid == 0) {
+					pb[ninst] = all;
+					ctx->total[c] = all;
+					ctx->ticket[c] = 0u;
+				}
 			}
-			if (tid < ninst) pl_base[tid] = x - nc;
-			const uint32_t all = __shfl(x, 63, 64);
-			if (tid == 0) {
-				pl_base[ninst] = all;
-				ctx->total = all;
-				ctx->ticket = 0u;
+			continue;
+		}
+		uint32_t carry = 0;
+		for (uint32_t b = 0; b < ninst; b += 1024u) {
+			uint32_t i = b + tid;
+			uint32_t nc = 0;
+			if (i < ninst) nc = chunks_of(i, c);
+			s_scan[tid] = nc;
+			__syncthreads();
+			for (uint32_t off = 1; off < 1024u; off <<= 1) {
+				uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
+				__syncthreads();
+				s_scan[tid] += v;
+				__syncthreads();
 			}
-		}
-		return;
-	}
-	uint32_t carry = 0;
-	for (uint32_t b = 0; b < ninst; b += 1024u) {
-		uint32_t i = b + tid;
-		uint32_t nc = 0;
-		if (i < ninst) nc = chunks_of(i);
-		s_scan[tid] = nc;
-		__syncthreads();
-		for (uint32_t off = 1; off < 1024u; off <<= 1) {
-			uint32_t v = tid >= off ? s_scan[tid - off] : 0u;
-			__syncthreads();
-			s_scan[tid] += v;
+			if (i < ninst) pb[i] = carry + s_scan[tid] - nc;
+			carry += s_scan[1023];
 			__syncthreads();
 		}
-		if (i < ninst) pl_base[i] = carry + s_scan[tid] - nc;
-		carry += s_scan[1023];
-		__syncthreads();
-	}
-	if (tid == 0) {
-		pl_base[ninst] = carry;
-		ctx->total = carry;
-		ctx->ticket = 0u;
+		if (tid == 0) {
+			pb[ninst] = carry;
+			ctx->total[c] = carry;
+			ctx->ticket[c] = 0u;
+		}
 	}
 }
 
 __global__ void __launch_bounds__(1024) k_plan(Inst *inst, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par,
-                                               uint32_t ninst)
+                                               uint32_t ninst, uint32_t ncls)
 {
-	plan_all(inst, ctx, pl_base, pl_par, ninst);
+	plan_all(inst, ctx, pl_base, pl_par, ninst, ncls);
 }
 
 /* Hold (hold[i] != 0) or resume instances 0..n-1, then plan the next pass.
  * Only running instances are held and only held ones resumed. */
 __global__ void __launch_bounds__(1024) k_hold(Inst *inst, const uint8_t *hold, uint32_t n, Ctx *ctx,
-                                               uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst)
+                                               uint32_t *pl_base, uint8_t *pl_par, uint32_t ninst, uint32_t ncls)
 {
 	for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
 		const uint32_t st = __hip_atomic_load(&inst[i].status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -562,7 +576,7 @@ __global__ void __launch_bounds__(1024) k_hold(Inst *inst, const uint8_t *hold, 
 		if (to != st) __hip_atomic_store(&inst[i].status, to, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	}
 	__syncthreads();
-	plan_all(inst, ctx, pl_base, pl_par, ninst);
+	plan_all(inst, ctx, pl_base, pl_par, ninst, ncls);
 }
 
 /* ------------------------------------------------------------ k_prep */
@@ -952,7 +966,8 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
  * publishes every work-group's record: no fences, no counter).  Work-groups
  * of slots with nothing to fold or begin (empty, held, finished) leave after
  * reading three words. */
-__global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par)
+__global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_t *pl_base, uint8_t *pl_par,
+                                                  uint32_t ncls)
 {
 	__shared__ Inst sI;
 	__shared__ uint32_t s_fold, s_begin, s_par, s_work;
@@ -1003,10 +1018,13 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 		const uint32_t par = sI.step & 1u;
 		const uint32_t nc = sI.status == SAMPO_RUNNING && sI.st[par].active ? sI.st[par].nchunk : 0u;
 		pl_par[0] = (uint8_t)par;
-		pl_base[0] = 0u;
-		pl_base[1] = nc;
-		ctx->total = nc;
-		ctx->ticket = 0u;
+		for (uint32_t c = 0; c < ncls; ++c) { /* one slot: its class has all chunks */
+			const uint32_t n = c == sI.tile_class ? nc : 0u;
+			pl_base[2u * c] = 0u;
+			pl_base[2u * c + 1u] = n;
+			ctx->total[c] = n;
+			ctx->ticket[c] = 0u;
+		}
 	}
 }
 
@@ -1032,8 +1050,8 @@ struct Src {
  * buffers.  Each lane's slots in a buffer are its own: it loads them, reads
  * them and writes its updated agent back into them. */
 template <int WG, int APT, int WAVES>
-__global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const uint32_t *pl_base,
-                                             const uint8_t *pl_par, uint32_t ninst)
+__global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const uint32_t *pl_base_all,
+                                             const uint8_t *pl_par, uint32_t ninst, uint32_t cls)
 {
 	constexpr int NW = WG / 64;
 	constexpr uint32_t TILE = (uint32_t)WG * APT;
@@ -1052,7 +1070,8 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 
 	const uint32_t tid = threadIdx.x, lane = tid & 63u;
 	const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6); /* scalar: addresses built on it need no VGPR */
-	const uint32_t total = ctx->total;
+	const uint32_t total = ctx->total[cls];
+	const uint32_t *const pl_base = pl_base_all + cls * (ninst + 1u); /* this class's tickets */
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 	uint32_t next = 0; /* global index of the segment holding the last issued tile's end */
 
@@ -1293,7 +1312,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				lds_barrier();
 				issue(ts2, te2, nxt, fin);
 			} else if (tid == 0) {
-				nt = gridDim.x + atomicAdd(&ctx->ticket, 1u);
+				nt = gridDim.x + atomicAdd(&ctx->ticket[cls], 1u);
 			}
 			uint32_t ns[APT], lr[APT];
 #pragma unroll 1
@@ -1514,13 +1533,14 @@ int fail(int code, const char *fmt, const char *a = "", long b = 0)
 
 struct Slot {
 	bool used = false;
+	uint32_t cls = 0;       /* its tile class */
 	void *mem = nullptr;    /* segments, chunks, counters, inputs, rows */
 	void *agents = nullptr; /* the two SoA agent buffers, 2 * NFIELD * P dwords */
 	uint32_t horizon = 0;
 	uint32_t P = 0;
 };
 
-typedef void (*pass_fn)(Inst *, Ctx *, const uint32_t *, const uint8_t *, uint32_t);
+typedef void (*pass_fn)(Inst *, Ctx *, const uint32_t *, const uint8_t *, uint32_t, uint32_t);
 
 template <int WG, int APT, int WAVES> pass_fn pass_ptr() { return k_pass<WG, APT, WAVES>; }
 
@@ -1559,11 +1579,18 @@ struct sampo_engine {
 	uint8_t *d_pl_par = nullptr;
 	uint8_t *d_hold = nullptr; /* sampo_engine_hold's mask */
 	std::vector<Slot> slots;
-	uint32_t wg = 768, apt = 1, waves = 6; /* two work-groups of 12 waves per CU */
+	/* tile classes: one pass launch per class each iteration, over the
+	 * instances admitted under it; class 0 starts as two work-groups of 12
+	 * waves per CU.  cur: the class of the next admissions. */
+	struct TileClass {
+		uint32_t wg = 768, apt = 1, waves = 6;
+		pass_fn pass = nullptr;
+		int grid = 0;
+		uint32_t n_inst = 0; /* instances admitted under it and not released */
+	} cls[MAX_CLASSES];
+	uint32_t ncls = 1, cur = 0;
 	uint32_t chunk = 4096;
 	uint32_t target_chunks = 0; /* 0: 2 per resident work-group of the pass */
-	pass_fn pass = nullptr;
-	int grid = 0;
 	uint32_t per_cu = 0; /* persistent work-groups per CU; 0: as many as fit */
 	hipGraphExec_t day = nullptr;
 	hipGraph_t day_graph = nullptr;
@@ -1596,14 +1623,15 @@ static void drop_graph(sampo_engine *e)
 	e->cap_ev.clear();
 }
 
-static int configure_pass(sampo_engine *e)
+static int configure_pass(sampo_engine *e, uint32_t c)
 {
 	int blocks = 0, cus = 0;
-	HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)e->pass, (int)e->wg, 0));
+	auto &k = e->cls[c];
+	HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void *)k.pass, (int)k.wg, 0));
 	HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
 	if (blocks < 1) blocks = 1;
 	if (e->per_cu && (int)e->per_cu < blocks) blocks = (int)e->per_cu;
-	e->grid = blocks * cus;
+	k.grid = blocks * cus;
 	return 0;
 }
 
@@ -1622,7 +1650,7 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 	HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
 	HIPCHK(hipMalloc(&e->d_inst, sizeof(Inst) * max_slots));
 	HIPCHK(hipMalloc(&e->d_ctx, sizeof(Ctx)));
-	HIPCHK(hipMalloc(&e->d_pl_base, sizeof(uint32_t) * (max_slots + 1)));
+	HIPCHK(hipMalloc(&e->d_pl_base, sizeof(uint32_t) * (max_slots + 1) * MAX_CLASSES));
 	HIPCHK(hipMalloc(&e->d_pl_par, max_slots));
 	HIPCHK(hipMalloc(&e->d_hold, max_slots));
 	HIPCHK(hipMemset(e->d_ctx, 0, sizeof(Ctx)));
@@ -1630,8 +1658,8 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 	memset(blank.data(), 0, sizeof(Inst) * max_slots);
 	for (auto &b : blank) b.status = SAMPO_EMPTY;
 	HIPCHK(hipMemcpy(e->d_inst, blank.data(), sizeof(Inst) * max_slots, hipMemcpyHostToDevice));
-	e->pass = pick_pass(e->wg, e->apt, e->waves);
-	int rc = configure_pass(e);
+	e->cls[0].pass = pick_pass(e->cls[0].wg, e->cls[0].apt, e->cls[0].waves);
+	int rc = configure_pass(e, 0);
 	if (rc) { sampo_engine_destroy(e); return rc; }
 	*out = e;
 	return 0;
@@ -1641,10 +1669,10 @@ extern "C" int sampo_engine_create(int device, uint32_t max_slots, sampo_engine 
 static void enqueue_prep(sampo_engine *e)
 {
 	hipLaunchKernelGGL(k_prep, dim3(e->nslots), dim3(PREP_WG), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base,
-	                   e->d_pl_par);
+	                   e->d_pl_par, e->ncls);
 	if (e->nslots > 1u)
 		hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
-		                   e->nslots);
+		                   e->nslots, e->ncls);
 }
 
 static void free_slot(Slot &s)
@@ -1675,14 +1703,30 @@ extern "C" int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt,
 	if (!e) return fail(SAMPO_E_ARG, "null engine%s%ld");
 	pass_fn f = pick_pass(wg, apt, waves);
 	if (!f) return fail(SAMPO_E_ARG, "tile shape not compiled in%s%ld");
+	for (uint32_t c = 0; c < e->ncls; ++c) /* a class of this shape: later admissions join it */
+		if (e->cls[c].wg == wg && e->cls[c].apt == apt && e->cls[c].waves == waves) {
+			e->cur = c;
+			return 0;
+		}
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
-	drop_graph(e);
-	e->wg = wg;
-	e->apt = apt;
-	e->waves = waves;
-	e->pass = f;
-	return configure_pass(e);
+	uint32_t c = e->ncls;
+	for (uint32_t q = 0; q < e->ncls && c == e->ncls; ++q) /* reuse a class no instance uses */
+		if (e->cls[q].n_inst == 0u) c = q;
+	if (c == MAX_CLASSES) return fail(SAMPO_E_FULL, "at most %s%ld tile shapes in use at once", "", (long)MAX_CLASSES);
+	if (c == e->ncls) { /* one more pass launch per iteration */
+		++e->ncls;
+		drop_graph(e);
+	} else if (e->cls[c].n_inst == 0u) {
+		drop_graph(e); /* the captured launch of that class has the old shape */
+	}
+	auto &k = e->cls[c];
+	k.wg = wg;
+	k.apt = apt;
+	k.waves = waves;
+	k.pass = f;
+	e->cur = c;
+	return configure_pass(e, c);
 }
 
 extern "C" int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu)
@@ -1692,7 +1736,11 @@ extern "C" int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu)
 	HIPCHK(hipStreamSynchronize(e->stream));
 	drop_graph(e);
 	e->per_cu = per_cu;
-	return configure_pass(e);
+	for (uint32_t c = 0; c < e->ncls; ++c) {
+		const int rc = configure_pass(e, c);
+		if (rc) return rc;
+	}
+	return 0;
 }
 
 extern "C" int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chunks)
@@ -1781,7 +1829,8 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	h.env_count = sc->env_count;
 	h.max_agents = sc->max_agents;
 	h.chunk_size = e->chunk;
-	h.target_chunks = e->target_chunks ? e->target_chunks : 2u * (uint32_t)e->grid;
+	h.target_chunks = e->target_chunks ? e->target_chunks : 2u * (uint32_t)e->cls[e->cur].grid;
+	h.tile_class = e->cur;
 	h.P_max = (uint32_t)round256(sc->max_agents);
 	sampo_species sp;
 	if (sc->species) sp = *sc->species;
@@ -1999,6 +2048,8 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 		return fail(SAMPO_E_HIP, "admission failed: %s%ld", hipGetErrorString(err));
 	}
 	S.used = true;
+	S.cls = e->cur;
+	e->cls[e->cur].n_inst += 1u;
 	e->slots[id] = S;
 	*slot_out = id;
 	return 0;
@@ -2020,9 +2071,10 @@ extern "C" int sampo_engine_release(sampo_engine *e, uint32_t slot)
 	}
 	/* plan the next pass without it before its memory goes */
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_ctx, e->d_pl_base, e->d_pl_par,
-	                   e->nslots);
+	                   e->nslots, e->ncls);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(e->stream));
+	e->cls[e->slots[slot].cls].n_inst -= 1u;
 	free_slot(e->slots[slot]);
 	return 0;
 }
@@ -2034,7 +2086,7 @@ extern "C" int sampo_engine_hold(sampo_engine *e, uint32_t n, const uint8_t *hol
 	if (n == 0) return 0;
 	HIPCHK(hipMemcpyAsync(e->d_hold, hold, n, hipMemcpyHostToDevice, e->stream));
 	hipLaunchKernelGGL(k_hold, dim3(1), dim3(1024), 0, e->stream, e->d_inst, e->d_hold, n, e->d_ctx, e->d_pl_base,
-	                   e->d_pl_par, e->nslots);
+	                   e->d_pl_par, e->nslots, e->ncls);
 	HIPCHK(hipGetLastError());
 	HIPCHK(hipStreamSynchronize(e->stream));
 	return 0;
@@ -2132,8 +2184,12 @@ static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after
 {
 	const unsigned int ef = capturing ? hipEventRecordExternal : hipEventRecordDefault;
 	if (before_pass) HIPCHK(hipEventRecordWithFlags(before_pass, e->stream, ef));
-	void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->nslots};
-	HIPCHK(hipLaunchKernel((const void *)e->pass, dim3(e->grid), dim3(e->wg), args, 0, e->stream));
+	for (uint32_t c = 0; c < e->ncls; ++c) { /* one pass per tile class, each over its own chunks */
+		uint32_t cc = c;
+		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->nslots, &cc};
+		HIPCHK(hipLaunchKernel((const void *)e->cls[c].pass, dim3(e->cls[c].grid), dim3(e->cls[c].wg), args, 0,
+		                       e->stream));
+	}
 	if (after_pass) HIPCHK(hipEventRecordWithFlags(after_pass, e->stream, ef));
 	enqueue_prep(e);
 	return 0;

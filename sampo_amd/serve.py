@@ -99,7 +99,6 @@ class Server:
             return time.perf_counter() - t_start if clock == "wall" else float(rounds)
 
         with S.Engine(max_slots=self.max_slots) as e, ThreadPoolExecutor(max_workers=self.threads) as pool:
-            e.set_tile(*self.tuning.tile)
             while True:
                 t = now()
                 while nxt < len(order) and specs[order[nxt]].arrival <= t:
@@ -125,6 +124,7 @@ class Server:
                                 self.log(f"serve: request {i} needs {need / 2**30:.1f} GiB, rejected")
                             continue
                         break
+                    e.set_tile(*self.tuning.tile_for(specs[i].n0))
                     e.set_chunk(self.tuning.chunk_for(specs[i].n0), NO_CHUNK_CAP)
                     try:
                         slot = e.admit(**kw)
@@ -231,7 +231,7 @@ def run_alone(spec, phys_factor=0.0, tuning=None):
     tuning = tuning or Tuning()
     kw = W.materialise(spec, phys_factor)
     with S.Engine(max_slots=1) as e:
-        e.set_tile(*tuning.tile)
+        e.set_tile(*tuning.tile_for(spec.n0))
         e.set_chunk(tuning.chunk_for(spec.n0), NO_CHUNK_CAP)
         slot = e.admit(**kw)
         while True:
