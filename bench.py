@@ -351,7 +351,7 @@ def main():
                     help="serve: advance the requests with the least work left first, or all every day")
     ap.add_argument("--min-active", type=int, default=64_000_000,
                     help="serve, srpt: agents kept in each day's step at least")
-    ap.add_argument("--tile", default="768x1x6", help="work-group size x agents per thread x min waves per SIMD")
+    ap.add_argument("--tile", default="512x1x6", help="work-group size x agents per thread x min waves per SIMD")
     ap.add_argument("--per-cu", type=int, default=0, help="persistent work-groups of the pass per CU (0: all that fit)")
     ap.add_argument("--chunk", type=int, default=0,
                     help="single run: minimum agents per chunk (0: 4096, or 512 below 1M initial agents, where "

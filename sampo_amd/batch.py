@@ -42,7 +42,7 @@ from sampo_amd import workloads as W
 
 BUCKET_BASE = 1e3
 CHUNK_CANDIDATES = (4096, 16384, 65536, 262144)
-TILE_CANDIDATES = ((768, 1, 6), (512, 1, 6), (256, 1, 8), (1024, 1, 6), (512, 2, 0), (256, 2, 6))
+TILE_CANDIDATES = ((512, 1, 6), (768, 1, 6), (256, 1, 8), (1024, 1, 6), (512, 2, 0), (256, 2, 6))
 MAX_TILE_SHAPES = 4  # pass launches per iteration the engine keeps (MAX_CLASSES)
 DEFAULT_CHUNK = 16384
 NO_CHUNK_CAP = 1 << 24  # target chunks per instance: effectively unlimited
@@ -58,7 +58,7 @@ def bucket_range(b):
 
 @dataclass
 class Tuning:
-    tile: tuple = (768, 1, 6)                    # shape of buckets without one of their own
+    tile: tuple = (512, 1, 6)                    # shape of buckets without one of their own
     chunk: dict = field(default_factory=dict)   # bucket -> agents per chunk
     tiles: dict = field(default_factory=dict)   # bucket -> tile shape
     table: list = field(default_factory=list)   # measurements

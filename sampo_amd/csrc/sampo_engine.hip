@@ -1580,10 +1580,10 @@ struct sampo_engine {
 	uint8_t *d_hold = nullptr; /* sampo_engine_hold's mask */
 	std::vector<Slot> slots;
 	/* tile classes: one pass launch per class each iteration, over the
-	 * instances admitted under it; class 0 starts as two work-groups of 12
+	 * instances admitted under it; class 0 starts as three work-groups of 8
 	 * waves per CU.  cur: the class of the next admissions. */
 	struct TileClass {
-		uint32_t wg = 768, apt = 1, waves = 6;
+		uint32_t wg = 512, apt = 1, waves = 6;
 		pass_fn pass = nullptr;
 		int grid = 0;
 		uint32_t n_inst = 0; /* instances admitted under it and not released */
