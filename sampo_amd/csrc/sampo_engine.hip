@@ -615,38 +615,41 @@ __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_
 }
 
 /* Exclusive scan of N values per thread over the work-group, all at once;
- * the totals land in tot (shared).  Waves scan with DPP; the wave totals of
- * column i are then scanned by row i % 4 of wave 0, 16 lanes for the 16
- * waves, four columns a round. */
+ * the totals land in tot (shared).  Waves scan with DPP and post their totals
+ * to LDS; after one barrier, lane i < N of every wave adds up column i of
+ * the waves before its own (its offset) and of all waves (the total), and
+ * the offsets reach the wave's lanes by readlane.  Column i of a wave is
+ * scanned only if bit i of `cols` (uniform over the wave) is set: the other
+ * columns hold zeros in that wave (with few chunks most waves hold none, and
+ * the scans are the work-group's VALU time). */
 template <int N>
-__device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot)
+__device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot, uint32_t cols)
 {
 	constexpr uint32_t NW = PREP_WG / 64u;
-	static_assert(NW <= 16u, "one 16-lane row holds a column's wave totals");
+	static_assert(N <= 64, "one lane per column");
 	__shared__ uint32_t lds[NW][N];
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+	cols = __builtin_amdgcn_readfirstlane(cols);
 #pragma unroll
 	for (int i = 0; i < N; ++i) { /* exclusive within the wave; lane 63 posts the wave total */
-		const uint32_t x = wave_incl_scan(v[i]);
+		uint32_t x = 0u;
+		if ((cols >> i) & 1u) x = wave_incl_scan(v[i]);
 		if (lane == 63u) lds[w][i] = x;
-		ex[i] = x - v[i];
+		ex[i] = (cols >> i) & 1u ? x - v[i] : 0u;
 	}
 	lds_barrier();
-	if (w == 0) {
-		const uint32_t q = lane & 15u, r = lane >> 4;
+	uint32_t off = 0u, all = 0u;
+	if (lane < (uint32_t)N) {
 #pragma unroll
-		for (uint32_t i0 = 0; i0 < (uint32_t)N; i0 += 4u) {
-			const uint32_t i = i0 + r;
-			const bool in = i < (uint32_t)N && q < NW;
-			const uint32_t y = in ? lds[q][i] : 0u;
-			const uint32_t s = row_incl_scan(y);
-			if (in) lds[q][i] = s - y;
-			if (q == 15u && i < (uint32_t)N) tot[i] = s;
+		for (uint32_t q = 0; q < NW; ++q) {
+			const uint32_t y = lds[q][lane];
+			all += y;
+			off += q < w ? y : 0u;
 		}
+		if (w == 0) tot[lane] = all;
 	}
-	lds_barrier();
 #pragma unroll
-	for (int i = 0; i < N; ++i) ex[i] += lds[w][i];
+	for (int i = 0; i < N; ++i) ex[i] += (uint32_t)__builtin_amdgcn_readlane((int)off, i);
 	lds_barrier();
 }
 
@@ -667,7 +670,14 @@ __device__ static inline uint32_t run_row(uint32_t sn, uint32_t kind)
  * thread's offset in every new state's range, and each thread writes its own
  * chunks' segments.  Only non-empty runs get a segment, so a tile of the
  * pass never overlaps more segments than it has agents (plus one); the scan
- * also counts each thread's non-empty runs per new state. */
+ * also counts each thread's non-empty runs per new state.
+ *
+ * With at most 512 chunks the work-group has threads to spare: TPC = 2, 4 or
+ * 8 parts of CW threads (whole waves) each take every chunk, part p only the
+ * new states s' with s' % TPC == p (and part 0 the sums).  A state's runs of
+ * a chunk are still all in one thread, so the scan's offsets are unchanged;
+ * each thread writes a TPC-th of the segments, and a wave scans only its
+ * parts' columns. */
 __device__ static void fold_iteration(Inst *I, uint32_t par)
 {
 	constexpr uint32_t KR = 2; /* chunks per thread held in registers */
@@ -681,7 +691,31 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	auto *nch = gmem(I->chunk[par ^ 1u]);
 	Seg *seg = I->seg[par ^ 1u]; /* written with gput */
 	const uint32_t K = (C + PREP_WG - 1u) / PREP_WG;
-	const uint32_t c_lo = tid * K < C ? tid * K : C, c_hi = c_lo + K < C ? c_lo + K : C;
+	const uint32_t CW = (C + 63u) & ~63u; /* threads of one part: whole waves */
+	const uint32_t spare = CW ? PREP_WG / CW : 1u;
+	const uint32_t TPC = spare >= 8u ? 8u : (spare >= 4u ? 4u : (spare >= 2u ? 2u : 1u));
+	uint32_t c_lo, c_hi, own; /* this thread's chunks and new states (bit s') */
+	if (TPC == 1u) {
+		c_lo = tid * K < C ? tid * K : C;
+		c_hi = c_lo + K < C ? c_lo + K : C;
+		own = 0xffu;
+	} else {
+		const uint32_t p = tid / CW, c = tid - p * CW;
+		const bool has = p < TPC && c < C;
+		c_lo = has ? c : C;
+		c_hi = has ? c + 1u : C;
+		own = 0u;
+		for (uint32_t sn = p; sn < 8u; sn += TPC) own |= 1u << sn;
+	}
+	const bool sums = TPC == 1u || tid < CW; /* part 0 (or every thread) adds the chunk sums */
+	/* the columns this wave can hold: counts and runs of its parts' states,
+	 * the sums in part 0; none in a wave without chunks */
+	uint32_t cols = 0u;
+	{
+		const uint32_t t0 = tid & ~63u; /* the wave's first thread */
+		const bool live = TPC == 1u ? t0 * K < C : (t0 / CW < TPC && t0 % CW < C);
+		if (live) cols = own | (own << 8) | (sums ? 0xff0000u : 0u);
+	}
 	const bool in_regs = K <= KR;
 	auto count = [&](uint32_t row, uint32_t c) -> uint32_t { return co[row * cmax + c]; };
 	/* loads: the run counts of this thread's chunks (when few) and their sums.
@@ -696,9 +730,11 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		for (uint32_t j = 0; j < KR; ++j) {
 			const uint32_t cj = c_lo + j < c_hi ? c_lo + j : 0u;
 #pragma unroll
-			for (uint32_t r = 0; r < 16u; ++r) (j ? nr1 : nr0)[r] = count(r, cj);
+			for (uint32_t r = 0; r < 16u; ++r) (j ? nr1 : nr0)[r] = count(r, cj); /* all rows: loads under a row mask measured slower */
+			if (sums) {
 #pragma unroll
-			for (uint32_t q = 0; q < 8u; ++q) v[q] += c_lo + j < c_hi ? co[(CO_SUMS + q) * cmax + cj] : 0u;
+				for (uint32_t q = 0; q < 8u; ++q) v[q] += c_lo + j < c_hi ? co[(CO_SUMS + q) * cmax + cj] : 0u;
+			}
 		}
 #pragma unroll
 		for (uint32_t r = 0; r < 16u; ++r) {
@@ -724,7 +760,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 #pragma unroll
 				for (uint32_t kind = 0; kind < 3u; ++kind) {
 					if (kind >= runs_of(sn)) continue;
-					const uint32_t n = (j ? nr1 : nr0)[run_row(sn, kind)]; /* zero past c_hi */
+					const uint32_t n = (own >> sn) & 1u ? (j ? nr1 : nr0)[run_row(sn, kind)] : 0u; /* zero past c_hi */
 					wv[sn] += n;
 					wv[8u + sn] += n ? 1u : 0u;
 				}
@@ -741,7 +777,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 				}
 	}
 	uint32_t ex[24];
-	block_excl_n<24>(wv, ex, s_tot);
+	block_excl_n<24>(wv, ex, s_tot, cols);
 	const uint32_t *s_sum = s_tot + 16; /* newEggs, bites, ..., potentially infective */
 	const uint32_t E = s_sum[0];
 	if (tid == 0) {
@@ -808,6 +844,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	if (in_regs) {
 #pragma unroll
 		for (uint32_t sn = 0; sn < 8u; ++sn) {
+			if (!((own >> sn) & 1u)) continue;
 			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
 			uint32_t idx = s_sb[sn] + ex[8u + sn];
 			uint32_t nxc = cut_at_or_after(cut, k0), nxk = cut_start(cut, nxc); /* next new chunk start at or after k0 */
@@ -838,9 +875,47 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	}
 }
 
+/* The per-iteration inputs begin_iteration reads from global memory, as
+ * words: gathered one word per lane of a wave before the fold, so that their
+ * round trip overlaps the fold's (BIN_* are word offsets). */
+enum {
+	BIN_TEMP = 0,                         /* temp[t] */
+	BIN_SR = 1,                           /* sprow[t] */
+	BIN_INC = BIN_SR + sizeof(SpRow) / 4, /* sprow[t - 1].inc: the newborns' laying temperature */
+	BIN_ENV = BIN_INC + 1,                /* env[t] (env[0] when there is one) */
+	BIN_SEED = BIN_ENV + sizeof(sampo_env) / 4, /* seed set step + 1 */
+	BIN_NBSEED = BIN_SEED + 16,           /* seeds[2], seeds[3] of set step */
+	BIN_WORDS = BIN_NBSEED + 8
+};
+static_assert(BIN_WORDS <= 64, "one lane per word");
+
+/* word q of the inputs of iteration `step` (the seeds only while step <
+ * horizon: both sets exist) */
+__device__ static inline uint32_t begin_input(const Inst *I, uint32_t step, uint32_t q)
+{
+	const uint32_t t = step < I->horizon ? step : I->horizon - 1u;
+	const uint32_t tp = step == 0u ? 0u : (step - 1u < I->horizon ? step - 1u : I->horizon - 1u);
+	const uint32_t te = I->env_count == 1u ? 0u : t;
+	const gu32 *p = nullptr;
+	if (q == BIN_TEMP)
+		p = (const gu32 *)I->temp + t;
+	else if (q < BIN_INC)
+		p = (const gu32 *)(I->sprow + t) + (q - BIN_SR);
+	else if (q == BIN_INC)
+		p = (const gu32 *)(I->sprow + tp) + offsetof(SpRow, inc) / 4u;
+	else if (q < BIN_SEED)
+		p = (const gu32 *)(I->env + te) + (q - BIN_ENV);
+	else if (q < BIN_NBSEED)
+		p = step < I->horizon ? (const gu32 *)I->seeds + (size_t)(step + 1u) * 16u + (q - BIN_SEED) : nullptr;
+	else if (q < BIN_WORDS)
+		p = step < I->horizon ? (const gu32 *)I->seeds + (size_t)step * 16u + 8u + (q - BIN_NBSEED) : nullptr;
+	return p ? *p : 0u;
+}
+
 /* Stage B: calcStats of the iteration in st[par] (src/abms.c:230-310), its
- * printStats row, checks, per-iteration parameters and chunks. */
-__device__ static void begin_iteration(Inst *I, uint32_t par)
+ * printStats row, checks, per-iteration parameters and chunks.  bin holds
+ * the words begin_input gives for this iteration. */
+__device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bin)
 {
 	__shared__ uint32_t s_status;
 	const uint32_t tid = threadIdx.x;
@@ -850,19 +925,17 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 		 * killAgents.cl's larval rate with L1DE and CC (reference mode), or
 		 * the stage rate (unused when the iteration does not run) */
 		const uint32_t d = tid - 128u;
-		const uint32_t t = ip.step < I->horizon ? ip.step : I->horizon - 1u;
-		const float cc = *gmem(&I->env[I->env_count == 1u ? 0u : t].carryingCapacity);
+		const float cc = sc_float(bin[BIN_ENV + offsetof(sampo_env, carryingCapacity) / 4u]);
 		ip.hmr_larva[d] = I->mode == SAMPO_MODE_REFERENCE ? sm_hmr_larva(d, I->l1de, cc) : I->hmr_stage[1];
 	}
 	if (tid >= 64u && tid < 88u && ip.step < I->horizon) {
 		/* the seed sets step + 1 (seed_set) and step (nb_set) of a running
-		 * iteration (step < horizon: both exist), loaded beside tid 0's work */
+		 * iteration (step < horizon: both exist), beside tid 0's work */
 		const uint32_t q = tid - 64u;
-		const gu32 *z = gmem(I->seeds);
 		if (q < 16u)
-			ip.seed[q] = z[(size_t)(ip.step + 1u) * 16u + q];
+			ip.seed[q] = bin[BIN_SEED + q];
 		else
-			ip.nbseed[q - 16u] = z[(size_t)ip.step * 16u + 8u + (q - 16u)];
+			ip.nbseed[q - 16u] = bin[BIN_NBSEED + (q - 16u)];
 	}
 	if (tid == 0) {
 		uint32_t n = 0;
@@ -898,16 +971,15 @@ __device__ static void begin_iteration(Inst *I, uint32_t par)
 			st = SAMPO_RUNNING;
 		}
 		gput(&I->rows[ip.step], row);
-		const uint32_t t = ip.step < I->horizon ? ip.step : I->horizon - 1u;
-		const uint32_t tp = ip.step == 0u ? 0u : (ip.step - 1u < I->horizon ? ip.step - 1u : I->horizon - 1u);
-		ip.temp = gmem(I->temp)[t];
-		const SpRow sr = gget(&I->sprow[t]);
-		ip.nb_inc = gmem(I->sprow)[tp].inc;
+		ip.temp = sc_float(bin[BIN_TEMP]);
+		SpRow sr;
+		for (uint32_t q = 0; q < sizeof(SpRow) / 4u; ++q) ((uint32_t *)&sr)[q] = bin[BIN_SR + q];
+		ip.nb_inc = sc_float(bin[BIN_INC]);
 		{
 			float q = 111.0f / (ip.temp - 16.0f);
 			ip.csd_inc = 1.0 / ((double)q * 14.0);
 		}
-		ip.env = gget(&I->env[I->env_count == 1u ? 0u : t]);
+		for (uint32_t q = 0; q < sizeof(sampo_env) / 4u; ++q) ((uint32_t *)&ip.env)[q] = bin[BIN_ENV + q];
 		ip.open = sr.open;
 		ip.dev_rate = sr.dev_rate;
 		ip.pupa_dly = sr.pupa_dly;
@@ -971,6 +1043,7 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 {
 	__shared__ Inst sI;
 	__shared__ uint32_t s_fold, s_begin, s_par, s_work;
+	__shared__ uint32_t s_bin[BIN_WORDS];
 	Inst *G = insts + blockIdx.x;
 	if (gridDim.x > 1u) {
 		if (threadIdx.x == 0)
@@ -994,6 +1067,12 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 	}
 	lds_barrier();
 	if (s_fold || s_begin) {
+		/* the inputs of the iteration about to begin, one word per lane of the
+		 * last wave, loaded before the fold so that both round trips overlap */
+		const uint32_t qb = threadIdx.x - (PREP_WG - 64u);
+		uint32_t bw = 0u;
+		if (threadIdx.x >= PREP_WG - 64u && qb < BIN_WORDS)
+			bw = begin_input(I, I->st[s_par].step + (s_fold ? 1u : 0u), qb);
 		if (s_fold) {
 			fold_iteration(I, s_par);
 			lds_barrier();
@@ -1004,9 +1083,10 @@ __global__ void __launch_bounds__(PREP_WG) k_prep(Inst *insts, Ctx *ctx, uint32_
 				s_par ^= 1u;
 			}
 		}
+		if (threadIdx.x >= PREP_WG - 64u && qb < BIN_WORDS) s_bin[qb] = bw;
 		lds_barrier();
 		if (s_begin) {
-			begin_iteration(I, s_par);
+			begin_iteration(I, s_par, s_bin);
 			if (threadIdx.x == 0) I->needs_begin = 0u;
 		}
 		lds_barrier();
@@ -1312,7 +1392,9 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				lds_barrier();
 				issue(ts2, te2, nxt, fin);
 			} else if (tid == 0) {
-				nt = gridDim.x + atomicAdd(&ctx->ticket[cls], 1u);
+				/* tickets past the first round: none when the grid covers every
+				 * chunk (then no work-group waits on the one shared counter) */
+				nt = total <= gridDim.x ? total : gridDim.x + atomicAdd(&ctx->ticket[cls], 1u);
 			}
 			uint32_t ns[APT], lr[APT];
 #pragma unroll 1
