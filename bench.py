@@ -27,8 +27,12 @@ sampo_amd/batch.py (per initial-size bucket), run before the timed region.
 Besides the JSON line's throughput, the single-run line carries:
   roofline     algorithmic bytes (34 B per agent in + 34 B per agent out) per
                fused-pass launch / that launch's duration from HIP events
-               recorded around every pass of the timed region (the timed
-               iterations run as one HIP graph, the events inside it);
+               recorded around every pass of the timed iterations run again
+               from the same inputs as one HIP graph with the events inside
+               it (the timed run itself is the product's path, one day graph
+               each with no events: an event pair costs an iteration about
+               7 us, a third of a small run's iteration; both runs must give
+               identical rows);
                traffic = HBM bytes per launch, the counter-measured ratio of
                traffic to algorithmic bytes (profiles/traffic_<workload>.json)
                times this run's algorithmic bytes per launch;
@@ -43,7 +47,7 @@ Besides the JSON line's throughput, the single-run line carries:
                workload's capacity: 5 bytes per agent / duration (HIP events).
 The default (large) line also carries the other halves of the metric, each
 as a sub-object measured in the same process:
-  small_run    the 100k-agent single run (launch-bound: its own throughput and
+  small_run    the 100k-agent single run (latency-bound: its own throughput and
                roofline fraction);
   batched      the 4096-scenario sweep in one engine against the plain
                one-scenario-at-a-time paths on a sample, with the ratio and
@@ -267,42 +271,61 @@ def serve_main(args):
 
 def single_run(w, args, name, steps, warmup, chunk):
     """One scenario in its own engine: (sub-object with throughput and
-    roofline, inputs resident)."""
+    roofline, inputs resident).  The scenario runs twice from the same
+    inputs: the timed days as the product runs them (one day graph each, no
+    events), then the same iterations again as one graph with an event pair
+    around each pass, for the pass durations.  Both runs must give identical
+    rows."""
     import sampo_amd as S
     temp, env, layout, agents = build_inputs(w)
     wg, apt, waves = (int(x) for x in (args.tile.split("x") + ["0"])[:3])
-    with S.Engine(max_slots=1) as e:
-        e.set_tile(wg, apt, waves)
-        e.set_grid(args.per_cu)
-        e.set_chunk(chunk, args.target_chunks)
-        t_admit = time.perf_counter()
-        slot = e.admit(mode=S.MODE_CONFIG, seed=w["seed"], horizon=24 * w["days"], temp=temp, env=env,
-                       max_agents=w["max_agents"], dmr=DMR, layout=layout, agents=agents)
-        e.sync()
-        admit_ms = (time.perf_counter() - t_admit) * 1e3
-        del agents
-        if warmup:
-            e.step(24 * warmup)
-        e.sync()
-        e.agent_steps(reset=True)
-        first = 24 * warmup
-        n_iter = 24 * steps
-        e.capture(n_iter)  # the timed iterations as one HIP graph, events around each pass (untimed)
-        e.sync()
-        t0 = time.perf_counter()
-        e.replay()
-        e.sync()
-        dt = time.perf_counter() - t0
-        wall_ms, pass_ms = e.replay_times()
-        status, step, n_now = e.status(slot)
-        if status not in (S.RUNNING, S.DONE) or step != first + n_iter:
-            sys.exit(f"instance stopped early: status {status} at iteration {step}")
-        steps_done = e.agent_steps()
-        rows = e.rows(slot, first, n_iter + 1)
+    first = 24 * warmup
+    n_iter = 24 * steps
+
+    def run(timed):
+        with S.Engine(max_slots=1) as e:
+            e.set_tile(wg, apt, waves)
+            e.set_grid(args.per_cu)
+            e.set_chunk(chunk, args.target_chunks)
+            t_admit = time.perf_counter()
+            slot = e.admit(mode=S.MODE_CONFIG, seed=w["seed"], horizon=24 * w["days"], temp=temp, env=env,
+                           max_agents=w["max_agents"], dmr=DMR, layout=layout, agents=agents)
+            e.sync()
+            admit_ms = (time.perf_counter() - t_admit) * 1e3
+            if warmup:
+                e.step(24 * warmup)
+            out = {"admit_ms": admit_ms}
+            if timed:
+                e.step_days(0)  # the day graph, captured before the timed region
+                e.sync()
+                e.agent_steps(reset=True)
+                t0 = time.perf_counter()
+                e.step_days(steps)  # the product's path: one day graph per day
+                e.sync()
+                out["dt"] = time.perf_counter() - t0
+                out["steps_done"] = e.agent_steps(reset=True)
+            else:
+                e.capture(n_iter)
+                e.sync()
+                e.replay()
+                e.sync()
+                out["wall_ms"], out["pass_ms"] = e.replay_times()
+            status, step, _ = e.status(slot)
+            if status not in (S.RUNNING, S.DONE) or step != first + n_iter:
+                sys.exit(f"instance stopped early: status {status} at iteration {step}")
+            out["rows"] = e.rows(slot, first, n_iter + 1)
+        return out
+
+    a = run(timed=True)
+    b = run(timed=False)
+    del agents
+    if not np.array_equal(a["rows"], b["rows"]):
+        sys.exit("the timed run and its replay with events differ")
+    dt, steps_done, rows = a["dt"], a["steps_done"], a["rows"]
+    wall_ms, pass_ms, admit_ms = b["wall_ms"], b["pass_ms"], a["admit_ms"]
     counts = rows["count"].astype(np.uint64).sum(axis=1)
     assert int(counts[:-1].sum()) == steps_done
-    alg_bytes = RECORD_BYTES * (int(counts[:-1].sum()) + int(counts[1:].sum()))
-    per_launch = alg_bytes / n_iter
+    per_launch = RECORD_BYTES * (int(counts[:-1].sum()) + int(counts[1:].sum())) / n_iter
     achieved = per_launch / (pass_ms / n_iter / 1e3) / 1e9
     ratio = traffic_ratio(name)
     value = steps_done / dt
@@ -311,7 +334,7 @@ def single_run(w, args, name, steps, warmup, chunk):
         "config": {"workload": w["desc"], "initial_agents": w["n0"], "max_agents": w["max_agents"],
                    "carrying_capacity": w["cc_per_agent"] * w["n0"], "timed_days": steps,
                    "iterations_per_day": 24, "tile": args.tile, "chunk": chunk,
-                   "target_chunks": args.target_chunks, "agents_at_end": n_now,
+                   "target_chunks": args.target_chunks, "agents_at_end": int(counts[-1]),
                    "mean_agents": steps_done / n_iter, "parallelism": "1 GPU"},
         "agent_days_per_s": value / 24.0,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -321,8 +344,10 @@ def single_run(w, args, name, steps, warmup, chunk):
                      # algorithmic one): their rate as a fraction of the peak
                      "traffic_frac": None if ratio is None else achieved * ratio / HBM_PEAK_GBS,
                      "kernel": "k_pass", "launches": n_iter,
+                     "measured_over": f"the {n_iter} timed iterations, run again from the same inputs (identical "
+                                      "rows) as one HIP graph with an event pair around each pass",
                      "algorithmic_bytes_per_launch": per_launch, "mean_launch_ms": pass_ms / n_iter},
-        "gpu_wall_ms_events": wall_ms,
+        "roofline_replay_wall_ms": wall_ms,
         "admit_ms": admit_ms,
     }
 

@@ -5,6 +5,8 @@ libsampo_icl.so     the reference's icl_* runtime API, scan API and kernel
                     contract over HIP, plus the reference driver loop and its
                     text input/output (csrc/sampo_icl.hip, csrc/sampo_abms.c,
                     csrc/sampo_report.c)
+grid_wait_probe     a measurement program: a grid-wide wait inside one launch
+                    against a kernel boundary in a graph (csrc/grid_wait_probe.hip)
 
 Every HIP source is compiled with -ffp-contract=off and IEEE division/sqrt so
 that device arithmetic follows the canonical numeric contract bit for bit,
@@ -30,6 +32,7 @@ TARGETS = {
     "libsampo_engine.so": ["sampo_engine.hip", "sampo_compact.hip", "sampo_species.c"],
     "libsampo_icl.so": ["sampo_icl.hip", "sampo_abms.c", "sampo_report.c", "sampo_species.c"],
 }
+PROGRAMS = {"grid_wait_probe": "grid_wait_probe.hip"}
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 
@@ -63,6 +66,11 @@ def build(verbose=False):
             objs.append(obj)
         dst = os.path.join(HERE, lib)
         _run(["hipcc", "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", dst, *objs], lib, verbose)
+        out.append(dst)
+    for exe, src in PROGRAMS.items():
+        dst = os.path.join(HERE, exe)
+        _run(["hipcc", *[f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")], "-o", dst, os.path.join(CSRC, src)],
+             exe, verbose)
         out.append(dst)
     return out
 
