@@ -39,6 +39,7 @@ extern "C" {
 #define SAMPO_PHYS_FULL 4u     /* physical slots of the instance exhausted */
 #define SAMPO_GROW 5u          /* paused: its buffers need more slots; sampo_engine_service() grows them */
 #define SAMPO_HELD 6u          /* paused by sampo_engine_hold() until it is resumed */
+#define SAMPO_FAULT 7u         /* a day launch's wait for its work-groups gave up; the instance's state is lost */
 #define SAMPO_EMPTY 0xffu      /* slot not in use */
 
 #define SAMPO_MODE_REFERENCE 0u /* killAgents.cl formulas, initial eggs via initAgents */
@@ -174,6 +175,20 @@ int sampo_engine_set_tile(sampo_engine *e, uint32_t wg, uint32_t apt, uint32_t w
 /* Persistent work-groups of the pass per CU (0: as many as the tile shape
  * lets reside; otherwise at most that many). */
 int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu);
+
+/* Day launches for instances admitted from now on (no counterpart in the
+ * reference, whose loop launches its kernels once per iteration,
+ * src/abms.c:502-529).  work_groups > 0: such an instance is cut into at
+ * most work_groups equal chunks, and while it is the only instance of the
+ * engine, sampo_engine_step() and sampo_engine_step_days() advance it by
+ * cooperative launches of work_groups work-groups, each running up to 24
+ * iterations (a simulated day) with no kernel boundary between them.  For
+ * instances small enough that a launch per iteration costs more than their
+ * agents (about 1M agents and below).  Results do not depend on it.
+ * work_groups must be at most 256 and at most the work-groups the device can
+ * keep resident at once (SAMPO_E_ARG), and the device must support
+ * cooperative launches (SAMPO_E_STATE).  0 (the default): off. */
+int sampo_engine_set_day(sampo_engine *e, uint32_t work_groups);
 
 /* Chunking of instances admitted from now on.  A chunk is the run of agents
  * one work-group takes and the granularity of output segments: at least

@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ENGINE_LIB = os.path.join(HERE, "libsampo_engine.so")
 
-RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, GROW, HELD, EMPTY = 0, 1, 2, 3, 4, 5, 6, 0xFF
+RUNNING, DONE, CAPACITY, EXTINCT, PHYS_FULL, GROW, HELD, FAULT, EMPTY = 0, 1, 2, 3, 4, 5, 6, 7, 0xFF
 MODE_REFERENCE, MODE_CONFIG = 0, 1
 STATES = ("egg", "larva", "pupa", "immature", "mateseeking", "bms", "bmd", "gravid")
 
@@ -102,6 +102,7 @@ def lib():
         L.sampo_engine_footprint.restype = C.c_uint64
         L.sampo_engine_poll.argtypes = [vp, C.c_uint32, vp, vp, vp]
         L.sampo_engine_service.argtypes = [vp, C.c_float, C.POINTER(C.c_uint32)]
+        L.sampo_engine_set_day.argtypes = [vp, C.c_uint32]
         L.sampo_engine_step.argtypes = [vp, C.c_uint32]
         L.sampo_engine_step_days.argtypes = [vp, C.c_uint32]
         L.sampo_engine_sync.argtypes = [vp]
@@ -321,6 +322,13 @@ class Engine:
     def set_grid(self, per_cu=0):
         """Persistent work-groups of the pass per CU (0: all that fit)."""
         _check(lib().sampo_engine_set_grid(self.h, per_cu))
+
+    def set_day(self, work_groups):
+        """Day launches for later admissions: such an instance is cut into at
+        most work_groups equal chunks and, while it is the engine's only
+        instance, advances by one cooperative launch per day (up to 24
+        iterations each).  0: off."""
+        _check(lib().sampo_engine_set_day(self.h, work_groups))
 
     def set_chunk(self, agents, target_chunks=0):
         """Minimum agents per chunk (work unit / output segment) and the most

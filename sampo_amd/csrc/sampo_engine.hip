@@ -39,6 +39,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <type_traits>
 #include <vector>
 
 #include "sampo_model.h"
@@ -48,6 +49,9 @@
 
 typedef __attribute__((address_space(3))) void lds_void;
 typedef __attribute__((address_space(1))) void glob_void;
+/* cache policy of a direct-to-LDS load: coherent at agent scope (sc1, the
+ * lowering of an agent-scope atomic load) */
+#define CPOL_SC1 16
 /* Global-memory views of pointers read from LDS: accesses through them are
  * global (not flat) instructions, which the compiler knows cannot touch LDS
  * -- so they never make it wait for global memory before an LDS access, and
@@ -230,6 +234,7 @@ struct Inst {
 	uint32_t status, step, pending, needs_begin;
 	uint32_t compact;      /* immature states store only their non-zero words (sm_fields) */
 	uint32_t tile_class;   /* the pass launch (tile shape) that advances it */
+	uint32_t even_cut;     /* cut into at most target_chunks equal chunks (day launches, chunk_cut) */
 	uint32_t l1de, females, potinf, ncyc, scyc, bites, ibites; /* stats of the start of `step` */
 	float hmr_stage[4];
 	float hmr_adult[100];
@@ -246,6 +251,33 @@ struct Ctx {
 	uint32_t ticket[MAX_CLASSES];
 	uint32_t total[MAX_CLASSES];
 	unsigned long long agent_steps;
+};
+
+/* Day launches (k_day): one cooperative launch runs up to DAY_ITERS
+ * iterations of a lone instance cut into at most DAY_CHUNKS equal chunks.
+ * Each work-group publishes its chunk's NOUT results as 64-bit words tagged
+ * with the iteration's sequence number (high half) in the parity of the
+ * iteration; every work-group waits for all of them and folds them itself.
+ * A tag is the launch's first sequence number plus the iteration within the
+ * launch, and the host never reuses one, so a word from another iteration or
+ * launch never matches.  Every wait gives up after DAY_POLLS polls: it sets
+ * `fault`, every other wait then gives up too, and the instance ends in
+ * SAMPO_FAULT.
+ *
+ * Memory order inside a launch: the agents one iteration writes and the next
+ * reads in another work-group, and the tagged words, all go through
+ * agent-scope accesses (sc1: coherent across the XCDs' L2 caches), so no L2
+ * write-back or invalidate is needed.  A work-group sends its tags only
+ * after every wave's agent stores have completed (s_waitcnt and a barrier),
+ * and reads agents only after it has seen every tag of the iteration. */
+#define DAY_CHUNKS 256u
+#define DAY_ITERS 24u
+#define DAY_SEGMAX (16u * DAY_CHUNKS + 2u) /* segments of an iteration with <= DAY_CHUNKS chunks */
+#define DAY_POLLS (1u << 20)
+
+struct DaySync {
+	unsigned long long w[2][NOUT][DAY_CHUNKS];
+	uint32_t fault;
 };
 
 /* ------------------------------------------------------------ the agent */
@@ -592,6 +624,11 @@ __global__ void __launch_bounds__(1024) k_hold(Inst *inst, const uint8_t *hold, 
 __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_t N)
 {
 	const uint32_t cs = I->chunk_size, T = I->target_chunks;
+	if (I->even_cut) { /* equal chunks of a multiple of 64 agents, at most T of them */
+		const uint32_t l = sm_roundup((uint32_t)(((uint64_t)N + T - 1u) / T), 64u);
+		const uint32_t L = l > 64u ? l : 64u;
+		return Cut{L, L, L, 0u, 0u};
+	}
 	Cut k = {cs, cs, cs, 0u, 0u};
 	if (!T) return k;
 	const uint32_t want = sm_roundup((uint32_t)(((uint64_t)N + T - 1u) / T), 256u);
@@ -622,10 +659,10 @@ __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_
  * scanned only if bit i of `cols` (uniform over the wave) is set: the other
  * columns hold zeros in that wave (with few chunks most waves hold none, and
  * the scans are the work-group's VALU time). */
-template <int N>
+template <int N, int NTH = PREP_WG>
 __device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot, uint32_t cols)
 {
-	constexpr uint32_t NW = PREP_WG / 64u;
+	constexpr uint32_t NW = NTH / 64u;
 	static_assert(N <= 64, "one lane per column");
 	__shared__ uint32_t lds[NW][N];
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -661,6 +698,46 @@ __device__ static inline uint32_t runs_of(uint32_t sn) { return sn == 0u ? 1u : 
 __device__ static inline uint32_t run_row(uint32_t sn, uint32_t kind)
 {
 	return sn == 0u ? 0u : (kind == 0u ? 8u + sn - 1u : (kind == 1u ? sn : 8u + S_GRAVID));
+}
+
+/* The serial part of a fold: from the totals tot (16 per-state counts of
+ * agents and of non-empty runs, then the 8 sums) of the finished iteration
+ * ip, the next one's counts, ranges, segment count, parity, step and chunk
+ * cut into np, the calcStats sums into I, and per new state its dense start
+ * kst, virtual start vst and first segment index sb.  Returns its agents. */
+__device__ static uint32_t fold_totals(Inst *I, const IterState &ip, IterState &np, const uint32_t *tot,
+                                       uint32_t *kst, uint32_t *vst, uint32_t *sb)
+{
+	const uint32_t *sum = tot + 16; /* newEggs, bites, ..., potentially infective */
+	const uint32_t E = sum[0];
+	uint32_t k = 0, vend = 0, b = E > 0u ? 1u : 0u;
+	for (uint32_t s = 0; s < 8u; ++s) {
+		const uint32_t c = tot[s] + (s == 0u ? E : 0u);
+		np.cnt[s] = c;
+		kst[s] = k;
+		k += c;
+		const uint32_t vs = s == 0 ? 0u : ((vend + SAMPO_PAD) / SAMPO_PAD) * SAMPO_PAD;
+		np.vstart[s] = vs;
+		vst[s] = vs;
+		vend = vs + c;
+		sb[s] = b;
+		b += tot[8 + s];
+	}
+	kst[8] = k;
+	np.nseg = b;
+	np.enew = E;
+	np.cur = ip.cur ^ 1u;
+	np.step = ip.step + 1u;
+	I->bites = sum[1];
+	I->ibites = sum[2];
+	I->ncyc = sum[3];
+	I->scyc = sum[4];
+	I->l1de = sum[5];
+	I->females = sum[6];
+	I->potinf = sum[7];
+	np.cut = chunk_cut(I, k);
+	np.seg0_done = 1u;
+	return k;
 }
 
 /* Stage A: fold the finished iteration of I (parity par) into the segment
@@ -778,40 +855,12 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 	}
 	uint32_t ex[24];
 	block_excl_n<24>(wv, ex, s_tot, cols);
-	const uint32_t *s_sum = s_tot + 16; /* newEggs, bites, ..., potentially infective */
-	const uint32_t E = s_sum[0];
+	const uint32_t E = s_tot[16]; /* newEggs */
 	if (tid == 0) {
-		uint32_t k = 0, vend = 0, sb = E > 0u ? 1u : 0u;
-		for (uint32_t s = 0; s < 8u; ++s) {
-			const uint32_t c = s_tot[s] + (s == 0u ? E : 0u);
-			np.cnt[s] = c;
-			s_kst[s] = k;
-			k += c;
-			uint32_t vs = s == 0 ? 0u : ((vend + SAMPO_PAD) / SAMPO_PAD) * SAMPO_PAD;
-			np.vstart[s] = vs;
-			s_vst[s] = vs;
-			vend = vs + c;
-			s_sb[s] = sb;
-			sb += s_tot[8 + s];
-		}
-		s_kst[8] = k;
-		np.nseg = sb;
-		np.enew = E;
-		np.cur = ip.cur ^ 1u;
-		np.step = ip.step + 1u;
-		I->bites = s_sum[1];
-		I->ibites = s_sum[2];
-		I->ncyc = s_sum[3];
-		I->scyc = s_sum[4];
-		I->l1de = s_sum[5];
-		I->females = s_sum[6];
-		I->potinf = s_sum[7];
-		const Cut nk = chunk_cut(I, k);
-		const uint32_t nc = cut_at_or_after(nk, k);
-		s_cut = nk;
+		const uint32_t k = fold_totals(I, ip, np, s_tot, s_kst, s_vst, s_sb);
+		const uint32_t nc = cut_at_or_after(np.cut, k);
+		s_cut = np.cut;
 		s_C = nc < cmax ? nc : cmax;
-		np.cut = nk;
-		np.seg0_done = 1u;
 	}
 	lds_barrier();
 	const Cut cut = s_cut, oc = ip.cut; /* chunks of the next and of the finished iteration */
@@ -913,9 +962,10 @@ __device__ static inline uint32_t begin_input(const Inst *I, uint32_t step, uint
 }
 
 /* Stage B: calcStats of the iteration in st[par] (src/abms.c:230-310), its
- * printStats row, checks, per-iteration parameters and chunks.  bin holds
- * the words begin_input gives for this iteration. */
-__device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bin)
+ * printStats row (stored when write_row is set), checks, per-iteration
+ * parameters and chunks.  bin holds the words begin_input gives for this
+ * iteration. */
+__device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bin, bool write_row = true)
 {
 	__shared__ uint32_t s_status;
 	const uint32_t tid = threadIdx.x;
@@ -970,7 +1020,7 @@ __device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bi
 		} else {
 			st = SAMPO_RUNNING;
 		}
-		gput(&I->rows[ip.step], row);
+		if (write_row) gput(&I->rows[ip.step], row);
 		ip.temp = sc_float(bin[BIN_TEMP]);
 		SpRow sr;
 		for (uint32_t q = 0; q < sizeof(SpRow) / 4u; ++q) ((uint32_t *)&sr)[q] = bin[BIN_SR + q];
@@ -1016,7 +1066,7 @@ __device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bi
 	/* initial populations: segment holding each chunk's first agent, segment
 	 * j holding the chunks whose start falls in [k0_j, k0_(j+1)) */
 	if (!have_seg0)
-		for (uint32_t j = tid; j < nseg && C; j += PREP_WG) {
+		for (uint32_t j = tid; j < nseg && C; j += blockDim.x) {
 			uint32_t a = seg[j].k0;
 			uint32_t z = j + 1u < nseg ? seg[j + 1u].k0 : N;
 			for (uint32_t c = cut_at_or_after(cut, a); c < C && cut_start(cut, c) < z; ++c) ch[c].seg0 = j;
@@ -1029,6 +1079,139 @@ __device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bi
 		I->status = st;
 		I->step = ip.step;
 	}
+}
+
+/* ------------------------------------------------------------ day launches */
+
+/* The last index i < m with k0[i] <= k, for k0 ascending with k0[0] <= k:
+ * the segment holding dense index k. */
+__device__ static inline uint32_t seg_find(const uint32_t *k0, uint32_t m, uint32_t k)
+{
+	uint32_t lo = 0, hi = m;
+	while (hi - lo > 1u) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (k0[mid] <= k) lo = mid; else hi = mid;
+	}
+	return lo;
+}
+
+/* The last index i in [lo, hi) with k0[i] <= k, for k0[lo] <= k, by one
+ * wave, 64 probes a round (the result in every lane). */
+__device__ static inline uint32_t seg_find_wave(const uint32_t *k0, uint32_t lo, uint32_t hi, uint32_t k)
+{
+	const uint32_t lane = threadIdx.x & 63u;
+	while (hi - lo > 1u) {
+		const uint32_t step = (hi - lo + 63u) / 64u;
+		const uint32_t j = lo + lane * step;
+		const uint64_t m64 = __ballot(j < hi && k0[j] <= k); /* lane 0 probes lo: never empty */
+		const uint32_t J = 63u - (uint32_t)__clzll((unsigned long long)m64);
+		const uint32_t nlo = lo + J * step;
+		hi = nlo + step < hi ? nlo + step : hi;
+		lo = nlo;
+	}
+	return lo;
+}
+
+/* The fold of a day launch, in every work-group: wait for every chunk's
+ * results of iteration j (words tagged `tag`, parity j & 1), then fold the
+ * finished iteration st[par] of the LDS record sI into st[par ^ 1] --
+ * fold_iteration's results -- and the new segment list into the LDS arrays
+ * k0/st/pb/vb (dense start, state | SEG_REV, physical base, virtual base).
+ * Thread c < C polls chunk c's NOUT words into registers, all at once and
+ * again only those not yet published, and places its runs after one scan
+ * over the work-group.  Returns false in every thread if a wait gave up. */
+template <int NTH>
+__device__ static bool day_fold(Inst &sI, uint32_t par, DaySync *sync, uint32_t j, uint32_t tag, uint32_t *sk0,
+                                uint32_t *sst, uint32_t *spb, uint32_t *svb)
+{
+	static_assert(NTH >= (int)DAY_CHUNKS, "a thread per chunk");
+	__shared__ uint32_t s_kst[9], s_vst[8], s_sb[8], s_tot[24];
+	const uint32_t tid = threadIdx.x;
+	const IterState &ip = sI.st[par];
+	IterState &np = sI.st[par ^ 1u];
+	const uint32_t C = ip.nchunk;
+	const bool mine = tid < C;
+	uint32_t v[NOUT];
+	{
+		unsigned long long *const base = &sync->w[j & 1u][0][tid];
+		uint32_t need = mine ? (1u << NOUT) - 1u : 0u;
+		bool gave_up = false;
+		for (uint32_t poll = 0; need; ++poll) {
+			unsigned long long x[NOUT];
+#pragma unroll
+			for (uint32_t q = 0; q < NOUT; ++q)
+				x[q] = (need >> q) & 1u ? __hip_atomic_load(base + q * DAY_CHUNKS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+				                        : 0ull;
+#pragma unroll
+			for (uint32_t q = 0; q < NOUT; ++q)
+				if (((need >> q) & 1u) && (uint32_t)(x[q] >> 32) == tag) {
+					v[q] = (uint32_t)x[q];
+					need &= ~(1u << q);
+				}
+			if (!need) break;
+			if (poll >= DAY_POLLS || __hip_atomic_load(&sync->fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+				__hip_atomic_store(&sync->fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				gave_up = true;
+				break;
+			}
+			__builtin_amdgcn_s_sleep(2);
+		}
+		/* every thread's words are in: the agents of the finished iteration
+		 * are read from here on, with agent-scope loads */
+		if (__syncthreads_or(gave_up)) return false;
+	}
+	uint32_t wv[24];
+#pragma unroll
+	for (uint32_t sn = 0; sn < 8u; ++sn) {
+		wv[sn] = 0u;
+		wv[8u + sn] = 0u;
+		wv[16u + sn] = mine ? v[CO_SUMS + sn] : 0u;
+#pragma unroll
+		for (uint32_t kind = 0; kind < 3u; ++kind) {
+			if (kind >= runs_of(sn)) continue;
+			const uint32_t n = mine ? v[run_row(sn, kind)] : 0u;
+			wv[sn] += n;
+			wv[8u + sn] += n ? 1u : 0u;
+		}
+	}
+	uint32_t ex[24];
+	block_excl_n<24, NTH>(wv, ex, s_tot, (tid & ~63u) < C ? 0xffffffu : 0u);
+	const uint32_t E = s_tot[16];
+	if (tid == 0) (void)fold_totals(&sI, ip, np, s_tot, s_kst, s_vst, s_sb);
+	lds_barrier();
+	if (tid == 0 && E > 0u) { /* the newborns */
+		sk0[0] = 0u;
+		sst[0] = 0u;
+		spb[0] = PB_NEWBORN;
+		svb[0] = 0u;
+	}
+	if (mine) {
+		const Cut oc = ip.cut;
+		const uint32_t ck0 = cut_start(oc, tid), ce = cut_start(oc, tid + 1u), ck1 = ce < ip.n ? ce : ip.n;
+#pragma unroll
+		for (uint32_t sn = 0; sn < 8u; ++sn) {
+			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
+			uint32_t idx = s_sb[sn] + ex[8u + sn];
+#pragma unroll
+			for (uint32_t kind = 0; kind < 3u; ++kind) {
+				if (kind >= runs_of(sn)) continue;
+				const uint32_t row = run_row(sn, kind), n = v[row];
+				if (!n) continue;
+				/* the source block: chunk tid's part of the state-src range */
+				const uint32_t src = row & 7u, mv = row >> 3;
+				const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
+				const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
+				sk0[idx] = k0;
+				sst[idx] = sn | (mv ? SEG_REV : 0u);
+				spb[idx] = mv ? be - 1u : bh;
+				svb[idx] = s_vst[sn] - s_kst[sn] + k0;
+				++idx;
+				k0 += n;
+			}
+		}
+	}
+	lds_barrier();
+	return true;
 }
 
 /* The instance record is worked on in LDS: the bookkeeping is a chain of
@@ -1129,13 +1312,28 @@ struct Src {
  * as direct-to-LDS loads (global_load_lds_dword) into the other of two LDS
  * buffers.  Each lane's slots in a buffer are its own: it loads them, reads
  * them and writes its updated agent back into them. */
-template <int WG, int APT, int WAVES>
+/* What a day launch adds to the pass's arguments. */
+struct DayArgs {
+	DaySync *sync;
+	uint32_t seq0, iters; /* the tag of its first iteration; iterations to run */
+	uint32_t *pl_base;    /* the plan it leaves for a pass that follows (one slot) */
+	uint8_t *pl_par;
+	uint32_t ncls;
+};
+
+/* The fused pass (DAY false): one iteration over every running instance of
+ * tile class cls, persistent work-groups taking chunks by ticket.  The day
+ * launch (DAY true, k_day below): the same chunk and tile code, fed chunks
+ * by work-group index in every iteration of the launch. */
+template <int WG, int APT, int WAVES, bool DAY = false>
 __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const uint32_t *pl_base_all,
-                                             const uint8_t *pl_par, uint32_t ninst, uint32_t cls)
+                                             const uint8_t *pl_par, uint32_t ninst, uint32_t cls, DayArgs da)
 {
 	constexpr int NW = WG / 64;
 	constexpr uint32_t TILE = (uint32_t)WG * APT;
-	constexpr uint32_t SEGC = TILE + 64u; /* staged segment window: a tile overlaps <= TILE + 1 */
+	/* the segment window: a tile overlaps <= TILE + 1 segments, staged from
+	 * the list in global memory; a day launch holds an iteration's whole list */
+	constexpr uint32_t SEGC = DAY ? DAY_SEGMAX : TILE + 64u;
 	__shared__ TileParams T;
 	__shared__ uint32_t s_ticket, s_inst, s_c;
 	__shared__ uint32_t s_k0[SEGC], s_st[SEGC], s_pb[SEGC], s_vb[SEGC];
@@ -1150,7 +1348,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 
 	const uint32_t tid = threadIdx.x, lane = tid & 63u;
 	const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6); /* scalar: addresses built on it need no VGPR */
-	const uint32_t total = ctx->total[cls];
+	const uint32_t total = DAY ? 0u : ctx->total[cls];
 	const uint32_t *const pl_base = pl_base_all + cls * (ninst + 1u); /* this class's tickets */
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 	uint32_t next = 0; /* global index of the segment holding the last issued tile's end */
@@ -1214,7 +1412,13 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 		uint32_t q = next - s_wlo;
 		while (q + 1u < m && s_k0[q + 1u] <= ts) ++q;
 		const uint32_t q_lo = q;
-		while (q + 1u < m && s_k0[q + 1u] < te) ++q;
+		/* the segment holding te - 1: a short walk, or in a day launch 64
+		 * probes a round when the tile spans many segments (fragmented
+		 * ranges: runs of a few agents) */
+		if (DAY && q + 8u < m && s_k0[q + 8u] < te)
+			q = seg_find_wave(s_k0, q, m, te - 1u);
+		else
+			while (q + 1u < m && s_k0[q + 1u] < te) ++q;
 		const uint32_t q_hi = q;
 		span_lo = s_st[q_lo] & 0xffu;
 		span_hi = s_st[q_hi] & 0xffu;
@@ -1226,7 +1430,15 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			s_ss[o] = S_DEAD;
 			if (k < te) {
 				uint32_t qq = q_lo;
-				for (uint32_t j = q_lo + 1u; j <= q_hi; ++j) qq += k >= s_k0[j] ? 1u : 0u;
+				if (!DAY || q_hi - q_lo <= 8u) { /* (the extra test measured 2 % slower in k_pass) */
+					for (uint32_t j = q_lo + 1u; j <= q_hi; ++j) qq += k >= s_k0[j] ? 1u : 0u;
+				} else { /* binary search among the tile's segments */
+					uint32_t hi = q_hi + 1u;
+					while (hi - qq > 1u) {
+						const uint32_t mid = (qq + hi) >> 1;
+						if (s_k0[mid] <= k) qq = mid; else hi = mid;
+					}
+				}
 				const uint32_t r = k - s_k0[qq];
 				const uint32_t pb = s_pb[qq], st = s_st[qq];
 				const bool born = pb == PB_NEWBORN;
@@ -1239,81 +1451,199 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					for (int f = 0; f < NFIELD; ++f)
 						if ((fm >> f) & 1u)
 							__builtin_amdgcn_global_load_lds((glob_void *)at_slot(fb[f], p),
-							                                 (lds_void *)&buf[f][(uint32_t)jx * WG + w * 64u], 4, 0, 0);
+							                                 (lds_void *)&buf[f][(uint32_t)jx * WG + w * 64u], 4, 0,
+							                                 DAY ? CPOL_SC1 : 0);
 				}
 			}
 		}
 	};
 
-	/* Tickets: a work-group's first chunk is its own index, later ones come
-	 * from the counter (from gridDim.x on), taken by tid 0 while the
-	 * chunk's last tile is worked on.  T keeps an instance's parameters
-	 * across its chunks: within a launch they do not change. */
+	/* Chunks.  k_pass takes them by ticket: a work-group's first chunk is its
+	 * own index, later ones come from the counter (from gridDim.x on), taken
+	 * by tid 0 while the chunk's last tile is worked on; T keeps an
+	 * instance's parameters across its chunks (within a launch they do not
+	 * change).  A day launch takes them by work-group index, iteration after
+	 * iteration; between two iterations every work-group folds the finished
+	 * one. */
 	uint32_t nt = blockIdx.x;
 	uint32_t have = 0xffffffffu;
+	/* a day launch: the lone instance in slot 0, its record in LDS in every
+	 * work-group, and the inputs of the iterations the launch may begin */
+	struct DayLds {
+		Inst sI;
+		uint32_t bin[DAY_ITERS][BIN_WORDS];
+	};
+	__shared__ std::conditional_t<DAY, DayLds, uint32_t> s_day;
+	uint32_t par = 0, iters = 0, j = 0; /* day launch: parity, iterations to run, the current one */
+	unsigned long long steps = 0;       /* day launch: agent-steps of the folded iterations */
+	bool fault = false;
+	/* a day launch: the record's iteration parameters into T */
+	auto load_params = [&](auto &dl) {
+		const IterState &ip = dl.sI.st[par];
+		const uint32_t *src = (const uint32_t *)&ip;
+		uint32_t *dst = (uint32_t *)&T.ip;
+		for (uint32_t q = tid; q < sizeof(IterState) / 4u; q += WG) dst[q] = src[q];
+		if (tid == 0) {
+			T.bin = dl.sI.buf[ip.cur];
+			T.bout = dl.sI.buf[ip.cur ^ 1u];
+			T.seg = dl.sI.seg[par];
+			T.chk = dl.sI.chunk[par];
+			T.nseg = ip.nseg;
+			T.cout = dl.sI.cout;
+			T.cmax = dl.sI.cmax;
+			T.P = dl.sI.P;
+			T.compact = dl.sI.compact;
+			s_wlo = 0u;
+			s_wm = ip.nseg;
+			s_wend = ip.n;
+		}
+	};
+	if constexpr (DAY) {
+		Inst &sI = s_day.sI;
+		{
+			const uint32_t *src = (const uint32_t *)insts;
+			uint32_t *dst = (uint32_t *)&sI;
+			for (uint32_t q = tid; q < sizeof(Inst) / 4u; q += WG) dst[q] = src[q];
+		}
+		__syncthreads();
+		par = sI.step & 1u;
+		iters = da.iters < DAY_ITERS ? da.iters : DAY_ITERS;
+		{
+			const IterState &ip = sI.st[par];
+			if (!(sI.status == SAMPO_RUNNING && ip.active)) return; /* nothing to run */
+			/* a day launch follows a prep (nothing pending or to begin) of an
+			 * instance cut into at most DAY_CHUNKS chunks */
+			if (sI.pending || sI.needs_begin || ip.nchunk > DAY_CHUNKS || ip.nseg > DAY_SEGMAX) {
+				if (blockIdx.x == 0 && tid == 0)
+					__hip_atomic_store(&insts->status, SAMPO_FAULT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				return;
+			}
+		}
+		load_params(s_day);
+		for (uint32_t q = tid; q < 100u; q += WG) T.hmr_adult[q] = sI.hmr_adult[q];
+		{ /* the first iteration's segment list into the window (later ones come
+		     from the folds), and the inputs of the iterations the launch may begin */
+			const gu32 *sg = (const gu32 *)sI.seg[par];
+			for (uint32_t i = tid; i < sI.st[par].nseg; i += WG) {
+				s_k0[i] = sg[4u * i];
+				s_st[i] = sg[4u * i + 1u];
+				s_pb[i] = sg[4u * i + 2u];
+				s_vb[i] = sg[4u * i + 3u];
+			}
+			const uint32_t step0 = sI.st[par].step;
+			for (uint32_t x = tid; x < iters * BIN_WORDS; x += WG) {
+				const uint32_t jj = x / BIN_WORDS;
+				s_day.bin[jj][x - jj * BIN_WORDS] = begin_input(&sI, step0 + 1u + jj, x - jj * BIN_WORDS);
+			}
+		}
+		__syncthreads();
+	}
 	for (;;) {
-		if (w == 0) { /* the instance holding ticket t: the last one whose base <= t, 64 probes a round */
-			const uint32_t t = __builtin_amdgcn_readfirstlane(nt);
-			uint32_t lo = 0, base = 0;
-			if (t < total) {
-				uint32_t hi = ninst;
-				while (hi - lo > 1u) {
-					const uint32_t step = (hi - lo + 63u) / 64u;
-					const uint32_t j = lo + lane * step;
-					const uint32_t b = j < hi ? pl_base[j] : 0xffffffffu; /* lane 0 probes lo: base <= t */
-					const uint64_t m = __ballot(b <= t);
-					const uint32_t J = 63u - (uint32_t)__clzll((unsigned long long)m);
-					base = __shfl(b, (int)J, 64);
-					const uint32_t nlo = lo + J * step;
-					hi = nlo + step < hi ? nlo + step : hi;
-					lo = nlo;
+		uint32_t ii, c;
+		if constexpr (DAY) {
+			if (nt >= T.ip.nchunk) { /* the iteration's chunks are done: fold it, begin the next */
+				Inst &sI = s_day.sI;
+				if (j + 1u == iters && blockIdx.x != 0u) return; /* work-group 0 alone folds the last */
+				if (!day_fold<WG>(sI, par, da.sync, j, da.seq0 + j, s_k0, s_st, s_pb, s_vb)) {
+					fault = true;
+					break;
+				}
+				steps += T.ip.n;
+				begin_iteration(&sI, par ^ 1u, s_day.bin[j], blockIdx.x == 0u);
+				lds_barrier();
+				par ^= 1u;
+				load_params(s_day);
+				lds_barrier();
+				if (sI.status != SAMPO_RUNNING || j + 1u == iters) {
+					if (blockIdx.x != 0u) return;
+					break;
+				}
+				if (T.ip.nchunk > DAY_CHUNKS) { /* the even cut keeps it within; never trust it blindly */
+					fault = true;
+					break;
+				}
+				nt = blockIdx.x;
+				++j;
+				continue;
+			}
+			ii = 0u;
+			c = nt;
+			nt += gridDim.x;
+		} else {
+			if (w == 0) { /* the instance holding ticket t: the last one whose base <= t, 64 probes a round */
+				const uint32_t t = __builtin_amdgcn_readfirstlane(nt);
+				uint32_t lo = 0, base = 0;
+				if (t < total) {
+					uint32_t hi = ninst;
+					while (hi - lo > 1u) {
+						const uint32_t step = (hi - lo + 63u) / 64u;
+						const uint32_t j = lo + lane * step;
+						const uint32_t b = j < hi ? pl_base[j] : 0xffffffffu; /* lane 0 probes lo: base <= t */
+						const uint64_t m = __ballot(b <= t);
+						const uint32_t J = 63u - (uint32_t)__clzll((unsigned long long)m);
+						base = __shfl(b, (int)J, 64);
+						const uint32_t nlo = lo + J * step;
+						hi = nlo + step < hi ? nlo + step : hi;
+						lo = nlo;
+					}
+				}
+				if (lane == 0) {
+					s_ticket = t;
+					s_inst = lo;
+					s_c = t - base;
 				}
 			}
-			if (lane == 0) {
-				s_ticket = t;
-				s_inst = lo;
-				s_c = t - base;
-			}
-		}
-		lds_barrier();
-		const uint32_t ticket = s_ticket;
-		if (ticket >= total) break;
-		const uint32_t ii = s_inst, c = s_c;
-		if (ii != have) { /* the instance's parameters into LDS; both parities are
-		                     loaded at once, with the parity, and one is kept */
-			have = ii;
-			const Inst *I = insts + ii;
-			const uint32_t par = pl_par[ii];
-			const uint32_t *s0 = (const uint32_t *)&I->st[0], *s1 = (const uint32_t *)&I->st[1];
-			uint32_t *dst = (uint32_t *)&T.ip;
-			for (uint32_t q = tid; q < sizeof(IterState) / 4u; q += WG) {
-				uint32_t x0 = s0[q], x1 = s1[q];
-				/* opaque values: the select must not become a load at a parity-dependent address */
-				__asm__("" : "+v"(x0), "+v"(x1));
-				dst[q] = par ? x1 : x0;
-			}
-			for (uint32_t q = tid; q < 100u; q += WG) T.hmr_adult[q] = I->hmr_adult[q];
-			if (tid == 0) {
-				uint32_t *b0 = I->buf[0], *b1 = I->buf[1];
-				const Seg *g0 = I->seg[0], *g1 = I->seg[1];
-				const Chunk *c0 = I->chunk[0], *c1 = I->chunk[1];
-				uint32_t cu0 = I->st[0].cur, cu1 = I->st[1].cur, ns0 = I->st[0].nseg, ns1 = I->st[1].nseg;
-				__asm__("" : "+v"(b0), "+v"(b1), "+v"(g0), "+v"(g1), "+v"(c0), "+v"(c1));
-				__asm__("" : "+v"(cu0), "+v"(cu1), "+v"(ns0), "+v"(ns1));
-				const uint32_t cur = par ? cu1 : cu0;
-				T.bin = cur ? b1 : b0;
-				T.bout = cur ? b0 : b1;
-				T.seg = par ? g1 : g0;
-				T.chk = par ? c1 : c0;
-				T.nseg = par ? ns1 : ns0;
-				T.cout = I->cout;
-				T.cmax = I->cmax;
-				T.P = I->P;
-				T.compact = I->compact;
-			}
 			lds_barrier();
+			const uint32_t ticket = s_ticket;
+			if (ticket >= total) break;
+			ii = s_inst;
+			c = s_c;
+			if (ii != have) { /* the instance's parameters into LDS; both parities are
+			                     loaded at once, with the parity, and one is kept */
+				have = ii;
+				const Inst *I = insts + ii;
+				const uint32_t par = pl_par[ii];
+				const uint32_t *s0 = (const uint32_t *)&I->st[0], *s1 = (const uint32_t *)&I->st[1];
+				uint32_t *dst = (uint32_t *)&T.ip;
+				for (uint32_t q = tid; q < sizeof(IterState) / 4u; q += WG) {
+					uint32_t x0 = s0[q], x1 = s1[q];
+					/* opaque values: the select must not become a load at a parity-dependent address */
+					__asm__("" : "+v"(x0), "+v"(x1));
+					dst[q] = par ? x1 : x0;
+				}
+				for (uint32_t q = tid; q < 100u; q += WG) T.hmr_adult[q] = I->hmr_adult[q];
+				if (tid == 0) {
+					uint32_t *b0 = I->buf[0], *b1 = I->buf[1];
+					const Seg *g0 = I->seg[0], *g1 = I->seg[1];
+					const Chunk *c0 = I->chunk[0], *c1 = I->chunk[1];
+					uint32_t cu0 = I->st[0].cur, cu1 = I->st[1].cur, ns0 = I->st[0].nseg, ns1 = I->st[1].nseg;
+					__asm__("" : "+v"(b0), "+v"(b1), "+v"(g0), "+v"(g1), "+v"(c0), "+v"(c1));
+					__asm__("" : "+v"(cu0), "+v"(cu1), "+v"(ns0), "+v"(ns1));
+					const uint32_t cur = par ? cu1 : cu0;
+					T.bin = cur ? b1 : b0;
+					T.bout = cur ? b0 : b1;
+					T.seg = par ? g1 : g0;
+					T.chk = par ? c1 : c0;
+					T.nseg = par ? ns1 : ns0;
+					T.cout = I->cout;
+					T.cmax = I->cmax;
+					T.P = I->P;
+					T.compact = I->compact;
+				}
+				lds_barrier();
+			}
 		}
-		if (tid == 0) { /* the chunk's range from the iteration's cut; its first segment from the record */
+		if constexpr (DAY) {
+			if (w == 0) { /* the chunk's range from the iteration's cut; its first segment from the window */
+				const uint32_t e = cut_start(T.ip.cut, c + 1u), ck0 = cut_start(T.ip.cut, c);
+				const uint32_t g = seg_find_wave(s_k0, 0u, s_wm, ck0);
+				if (lane == 0) {
+					T.ch.k0 = ck0;
+					T.ch.k1 = e < T.ip.n ? e : T.ip.n;
+					T.ch.seg0 = g;
+				}
+			}
+		} else if (tid == 0) { /* the chunk's range from the iteration's cut; its first segment from the record */
 			const uint32_t e = cut_start(T.ip.cut, c + 1u);
 			T.ch.k0 = cut_start(T.ip.cut, c);
 			T.ch.k1 = e < T.ip.n ? e : T.ip.n;
@@ -1345,7 +1675,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 		}
 		/* prologue: tile 0's window and loads */
 		next = T.ch.seg0;
-		{
+		if constexpr (!DAY) {
 			const uint32_t te0 = k0 + TILE < k1 ? k0 + TILE : k1;
 			if (w == 0) stage(next, te0);
 			lds_barrier();
@@ -1388,10 +1718,12 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			const bool more = ts2 < k1;
 			lds_barrier(); /* every lane has read the window for this tile */
 			if (more) {
-				if (w == 0 && te2 > s_wend) stage(next, te2);
-				lds_barrier();
+				if constexpr (!DAY) { /* a day launch's window holds the whole list */
+					if (w == 0 && te2 > s_wend) stage(next, te2);
+					lds_barrier();
+				}
 				issue(ts2, te2, nxt, fin);
-			} else if (tid == 0) {
+			} else if (!DAY && tid == 0) {
 				/* tickets past the first round: none when the grid covers every
 				 * chunk (then no work-group waits on the one shared counter) */
 				nt = total <= gridDim.x ? total : gridDim.x + atomicAdd(&ctx->ticket[cls], 1u);
@@ -1455,7 +1787,13 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t fm = sm_fields(sn > S_GRAVID ? S_BMS : sn, T.compact);
 #pragma unroll
 					for (int f = 0; f < NFIELD; ++f)
-						if ((fm >> f) & 1u) *at_slot(fout[f], p) = cur[f][o];
+						if ((fm >> f) & 1u) {
+							if constexpr (DAY) /* coherent at agent scope: other work-groups read it next */
+								__hip_atomic_store((uint32_t *)at_slot(fout[f], p), cur[f][o], __ATOMIC_RELAXED,
+								                   __HIP_MEMORY_SCOPE_AGENT);
+							else
+								*at_slot(fout[f], p) = cur[f][o];
+						}
 				}
 			}
 			/* the next tile rewrites s_wc only after its first barrier */
@@ -1473,19 +1811,81 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				uint32_t r = wave_sum(v[q]);
 				if (lane == 0) s_acc[w][q] = r;
 			}
-			lds_barrier();
-			if (tid < 8u) { /* the sums in cout order: the NEV events, then l1de, fem, pinf */
-				uint32_t r = 0;
-				for (int q = 0; q < NW; ++q) r += tid < NEV ? s_ev[q][tid] : s_acc[q][tid - NEV];
-				((gu32 *)T.cout)[(size_t)(CO_SUMS + tid) * T.cmax + c] = r;
+			if constexpr (DAY) {
+				/* published once the chunk's agents are: every wave waits until its
+				 * agent stores are complete (agent-scope stores, visible to every
+				 * work-group once complete), then wave 0 sends the tagged words,
+				 * rows in cout order */
+				__asm__ __volatile__("" ::: "memory");
+				__builtin_amdgcn_s_waitcnt(0);
+				__syncthreads();
+				if (w == 0) {
+					if (lane < NOUT) {
+						uint32_t r = 0;
+						if (lane < 16u)
+							r = s_run[2u * (lane & 7u) + (lane >> 3)];
+						else
+							for (int q = 0; q < NW; ++q)
+								r += lane - 16u < NEV ? s_ev[q][lane - 16u] : s_acc[q][lane - 16u - NEV];
+						__hip_atomic_store(&da.sync->w[j & 1u][lane][c], ((unsigned long long)(da.seq0 + j) << 32) | r,
+						                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+					}
+				}
+			} else {
+				lds_barrier();
+				if (tid < 8u) { /* the sums in cout order: the NEV events, then l1de, fem, pinf */
+					uint32_t r = 0;
+					for (int q = 0; q < NW; ++q) r += tid < NEV ? s_ev[q][tid] : s_acc[q][tid - NEV];
+					((gu32 *)T.cout)[(size_t)(CO_SUMS + tid) * T.cmax + c] = r;
+				}
+				if (tid < 16u) /* stayed: row q, moved on: row 8 + q */
+					((gu32 *)T.cout)[(size_t)((tid & 1u) * 8u + (tid >> 1)) * T.cmax + c] = s_run[tid];
+				if (tid == 0) insts[ii].pending = 1u;
 			}
-			if (tid < 16u) /* stayed: row q, moved on: row 8 + q */
-				((gu32 *)T.cout)[(size_t)((tid & 1u) * 8u + (tid >> 1)) * T.cmax + c] = s_run[tid];
-			if (tid == 0) insts[ii].pending = 1u;
 		}
 		lds_barrier();
 	}
+	if constexpr (DAY) {
+		Inst &sI = s_day.sI;
+		if (blockIdx.x != 0u) return;
+		if (fault) {
+			if (tid == 0) __hip_atomic_store(&insts->status, SAMPO_FAULT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			return;
+		}
+		/* work-group 0 leaves what a prep would: the next iteration's segment
+		 * list and chunk starts, the record, the plan of a following pass */
+		const IterState &np = sI.st[par];
+		{
+			gu32x4 *sg = (gu32x4 *)sI.seg[par];
+			for (uint32_t i = tid; i < np.nseg; i += WG) sg[i] = u32x4{s_k0[i], s_st[i], s_pb[i], s_vb[i]};
+			auto *ch = gmem(sI.chunk[par]);
+			for (uint32_t c = tid; c < np.nchunk; c += WG) ch[c].seg0 = seg_find(s_k0, np.nseg, cut_start(np.cut, c));
+			const uint32_t *src = (const uint32_t *)&sI;
+			gu32 *dst = (gu32 *)insts;
+			for (uint32_t q = tid; q < sizeof(Inst) / 4u; q += WG) dst[q] = src[q];
+		}
+		if (tid == 0) {
+			const uint32_t nc = sI.status == SAMPO_RUNNING && np.active ? np.nchunk : 0u;
+			da.pl_par[0] = (uint8_t)par;
+			for (uint32_t c = 0; c < da.ncls; ++c) {
+				const uint32_t n = c == sI.tile_class ? nc : 0u;
+				da.pl_base[2u * c] = 0u;
+				da.pl_base[2u * c + 1u] = n;
+				ctx->total[c] = n;
+				ctx->ticket[c] = 0u;
+			}
+			atomicAdd(&ctx->agent_steps, steps);
+		}
+	}
 }
+
+/* A day launch: up to DAY_ITERS iterations of the lone instance in slot 0
+ * in one cooperative launch (every work-group resident), a work-group per
+ * chunk of its even cut.  Between iterations every work-group gathers all
+ * chunk results and folds them itself, in place of two kernel boundaries and
+ * a one-work-group prep; work-group 0 leaves the state a prep would. */
+#define DAY_WG 512
+static const void *k_day() { return (const void *)k_pass<DAY_WG, 1, 0, true>; }
 
 /* ------------------------------------------------------------ export, histogram */
 
@@ -1616,13 +2016,14 @@ int fail(int code, const char *fmt, const char *a = "", long b = 0)
 struct Slot {
 	bool used = false;
 	uint32_t cls = 0;       /* its tile class */
+	uint32_t day = 0;       /* work-groups of its day launches (its even cut); 0: none */
 	void *mem = nullptr;    /* segments, chunks, counters, inputs, rows */
 	void *agents = nullptr; /* the two SoA agent buffers, 2 * NFIELD * P dwords */
 	uint32_t horizon = 0;
 	uint32_t P = 0;
 };
 
-typedef void (*pass_fn)(Inst *, Ctx *, const uint32_t *, const uint8_t *, uint32_t, uint32_t);
+typedef void (*pass_fn)(Inst *, Ctx *, const uint32_t *, const uint8_t *, uint32_t, uint32_t, DayArgs);
 
 template <int WG, int APT, int WAVES> pass_fn pass_ptr() { return k_pass<WG, APT, WAVES>; }
 
@@ -1676,6 +2077,11 @@ struct sampo_engine {
 	uint32_t per_cu = 0; /* persistent work-groups per CU; 0: as many as fit */
 	hipGraphExec_t day = nullptr;
 	hipGraph_t day_graph = nullptr;
+	/* day launches (k_day): work-groups for later admissions (0: off), the
+	 * tagged words, the next tag */
+	uint32_t day_wg = 0;
+	DaySync *d_day = nullptr;
+	uint32_t day_seq = 1;
 	/* sampo_engine_capture: n iterations with an event pair around each pass */
 	hipGraphExec_t cap = nullptr;
 	hipGraph_t cap_graph = nullptr;
@@ -1776,6 +2182,7 @@ extern "C" void sampo_engine_destroy(sampo_engine *e)
 	(void)hipFree(e->d_pl_base);
 	(void)hipFree(e->d_pl_par);
 	(void)hipFree(e->d_hold);
+	if (e->d_day) (void)hipFree(e->d_day);
 	if (e->stream) (void)hipStreamDestroy(e->stream);
 	delete e;
 }
@@ -1822,6 +2229,28 @@ extern "C" int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu)
 		const int rc = configure_pass(e, c);
 		if (rc) return rc;
 	}
+	return 0;
+}
+
+extern "C" int sampo_engine_set_day(sampo_engine *e, uint32_t work_groups)
+{
+	if (!e || work_groups > DAY_CHUNKS) return fail(SAMPO_E_ARG, "work_groups must be 0..%s%ld", "", (long)DAY_CHUNKS);
+	if (work_groups) {
+		int coop = 0, blocks = 0, cus = 0;
+		HIPCHK(hipSetDevice(e->device));
+		HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, e->device));
+		if (!coop) return fail(SAMPO_E_STATE, "device has no cooperative launch%s%ld");
+		HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_day(), DAY_WG, 0));
+		HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+		if ((long)blocks * cus < (long)work_groups)
+			return fail(SAMPO_E_ARG, "at most %s%ld work-groups of a day launch fit at once", "", (long)blocks * cus);
+		if (!e->d_day) {
+			HIPCHK(hipStreamSynchronize(e->stream));
+			HIPCHK(hipMalloc(&e->d_day, sizeof(DaySync)));
+			HIPCHK(hipMemset(e->d_day, 0, sizeof(DaySync)));
+		}
+	}
+	e->day_wg = work_groups;
 	return 0;
 }
 
@@ -2023,6 +2452,10 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	const SlotLayout L = slot_layout(h.P_max, e->chunk, sc->horizon, sc->env_count);
 	h.cmax = L.cmax;
 	h.smax = L.smax;
+	if (e->day_wg) { /* day launches: a work-group per chunk, as many chunks as the counters hold */
+		h.even_cut = 1u;
+		h.target_chunks = e->day_wg < L.cmax - 2u ? e->day_wg : L.cmax - 2u;
+	}
 
 	HIPCHK(hipSetDevice(e->device));
 	HIPCHK(hipStreamSynchronize(e->stream));
@@ -2131,6 +2564,7 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	}
 	S.used = true;
 	S.cls = e->cur;
+	S.day = h.even_cut ? e->day_wg : 0u;
 	e->cls[e->cur].n_inst += 1u;
 	e->slots[id] = S;
 	*slot_out = id;
@@ -2268,7 +2702,8 @@ static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after
 	if (before_pass) HIPCHK(hipEventRecordWithFlags(before_pass, e->stream, ef));
 	for (uint32_t c = 0; c < e->ncls; ++c) { /* one pass per tile class, each over its own chunks */
 		uint32_t cc = c;
-		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->nslots, &cc};
+		DayArgs none = {};
+		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &e->nslots, &cc, &none};
 		HIPCHK(hipLaunchKernel((const void *)e->cls[c].pass, dim3(e->cls[c].grid), dim3(e->cls[c].wg), args, 0,
 		                       e->stream));
 	}
@@ -2277,8 +2712,36 @@ static int enqueue_one(sampo_engine *e, hipEvent_t before_pass, hipEvent_t after
 	return 0;
 }
 
+/* The lone instance advances by day launches: it is in slot 0 (the
+ * launches cover one slot) and was admitted for them. */
+static bool day_mode(const sampo_engine *e)
+{
+	return e->nslots == 1u && e->slots[0].used && e->slots[0].day && e->d_day;
+}
+
+/* n iterations as day launches of at most DAY_ITERS iterations each */
+static int enqueue_days(sampo_engine *e, uint32_t n)
+{
+	const uint32_t G = e->slots[0].day;
+	while (n) {
+		const uint32_t it = n < DAY_ITERS ? n : DAY_ITERS;
+		if (e->day_seq > 0xffffffffu - DAY_ITERS) { /* tags are never reused: clear them and start over */
+			HIPCHK(hipMemsetAsync(e->d_day->w, 0, sizeof(e->d_day->w), e->stream));
+			e->day_seq = 1u;
+		}
+		DayArgs da = {e->d_day, e->day_seq, it, e->d_pl_base, e->d_pl_par, e->ncls};
+		uint32_t one = 1u, cls0 = 0u;
+		void *args[] = {&e->d_inst, &e->d_ctx, &e->d_pl_base, &e->d_pl_par, &one, &cls0, &da};
+		HIPCHK(hipLaunchCooperativeKernel(k_day(), dim3(G), dim3(DAY_WG), args, 0, e->stream));
+		e->day_seq += it;
+		n -= it;
+	}
+	return 0;
+}
+
 static int enqueue(sampo_engine *e, uint32_t n)
 {
+	if (day_mode(e)) return enqueue_days(e, n);
 	for (uint32_t i = 0; i < n; ++i) {
 		int rc = enqueue_one(e, nullptr, nullptr);
 		if (rc) return rc;
@@ -2298,6 +2761,7 @@ extern "C" int sampo_engine_step_days(sampo_engine *e, uint32_t days)
 {
 	if (!e) return fail(SAMPO_E_ARG, "null engine%s%ld");
 	HIPCHK(hipSetDevice(e->device));
+	if (day_mode(e)) return enqueue_days(e, 24u * days);
 	if (!e->day) {
 		HIPCHK(hipStreamSynchronize(e->stream));
 		HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
