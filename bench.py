@@ -269,13 +269,14 @@ def serve_main(args):
         sys.exit("not every request completed")
 
 
-def single_run(w, args, name, steps, warmup, chunk):
+def single_run(w, args, name, steps, warmup, chunk, day=0):
     """One scenario in its own engine: (sub-object with throughput and
     roofline, inputs resident).  The scenario runs twice from the same
-    inputs: the timed days as the product runs them (one day graph each, no
-    events), then the same iterations again as one graph with an event pair
-    around each pass, for the pass durations.  Both runs must give identical
-    rows."""
+    inputs: the timed days as the product runs them (one day graph each, or
+    with day > 0 one cooperative day launch of `day` work-groups each, no
+    events), then the same days again with HIP events: around each pass of
+    the iterations replayed as one graph, or around each day launch.  Both
+    runs must give identical rows."""
     import sampo_amd as S
     temp, env, layout, agents = build_inputs(w)
     wg, apt, waves = (int(x) for x in (args.tile.split("x") + ["0"])[:3])
@@ -287,6 +288,7 @@ def single_run(w, args, name, steps, warmup, chunk):
             e.set_tile(wg, apt, waves)
             e.set_grid(args.per_cu)
             e.set_chunk(chunk, args.target_chunks)
+            e.set_day(day)
             t_admit = time.perf_counter()
             slot = e.admit(mode=S.MODE_CONFIG, seed=w["seed"], horizon=24 * w["days"], temp=temp, env=env,
                            max_agents=w["max_agents"], dmr=DMR, layout=layout, agents=agents)
@@ -304,6 +306,8 @@ def single_run(w, args, name, steps, warmup, chunk):
                 e.sync()
                 out["dt"] = time.perf_counter() - t0
                 out["steps_done"] = e.agent_steps(reset=True)
+            elif day:
+                out["wall_ms"], out["pass_ms"] = e.time_days(steps)
             else:
                 e.capture(n_iter)
                 e.sync()
@@ -325,15 +329,17 @@ def single_run(w, args, name, steps, warmup, chunk):
     wall_ms, pass_ms, admit_ms = b["wall_ms"], b["pass_ms"], a["admit_ms"]
     counts = rows["count"].astype(np.uint64).sum(axis=1)
     assert int(counts[:-1].sum()) == steps_done
-    per_launch = RECORD_BYTES * (int(counts[:-1].sum()) + int(counts[1:].sum())) / n_iter
-    achieved = per_launch / (pass_ms / n_iter / 1e3) / 1e9
-    ratio = traffic_ratio(name)
+    launches = steps if day else n_iter
+    per_launch = RECORD_BYTES * (int(counts[:-1].sum()) + int(counts[1:].sum())) / launches
+    achieved = per_launch / (pass_ms / launches / 1e3) / 1e9
+    ratio = None if day else traffic_ratio(name)
     value = steps_done / dt
     return {
         "value": value, "steps": steps, "warmup": warmup, "ms_per_step": dt * 1e3 / steps,
         "config": {"workload": w["desc"], "initial_agents": w["n0"], "max_agents": w["max_agents"],
                    "carrying_capacity": w["cc_per_agent"] * w["n0"], "timed_days": steps,
-                   "iterations_per_day": 24, "tile": args.tile, "chunk": chunk,
+                   "iterations_per_day": 24, "tile": "512x1 (day launch)" if day else args.tile,
+                   "day_launch_work_groups": day, "chunk": chunk,
                    "target_chunks": args.target_chunks, "agents_at_end": int(counts[-1]),
                    "mean_agents": steps_done / n_iter, "parallelism": "1 GPU"},
         "agent_days_per_s": value / 24.0,
@@ -343,13 +349,31 @@ def single_run(w, args, name, steps, warmup, chunk):
                      # the bytes the pass really moves (the record is stored in fewer bytes than the 34-B
                      # algorithmic one): their rate as a fraction of the peak
                      "traffic_frac": None if ratio is None else achieved * ratio / HBM_PEAK_GBS,
-                     "kernel": "k_pass", "launches": n_iter,
-                     "measured_over": f"the {n_iter} timed iterations, run again from the same inputs (identical "
-                                      "rows) as one HIP graph with an event pair around each pass",
-                     "algorithmic_bytes_per_launch": per_launch, "mean_launch_ms": pass_ms / n_iter},
+                     "kernel": "k_pass<512, 1, 0, true> (day launch: 24 iterations, passes and folds)" if day
+                               else "k_pass", "launches": launches,
+                     "measured_over": (f"the {steps} timed days, run again from the same inputs (identical rows), "
+                                       "an event pair around each day launch") if day else
+                                      (f"the {n_iter} timed iterations, run again from the same inputs (identical "
+                                       "rows) as one HIP graph with an event pair around each pass"),
+                     "algorithmic_bytes_per_launch": per_launch, "mean_launch_ms": pass_ms / launches},
         "roofline_replay_wall_ms": wall_ms,
         "admit_ms": admit_ms,
     }
+
+
+def day_groups(args, w):
+    """Work-groups of the day launches of a single run: --day, or 256 (one
+    per CU) below 1M initial agents, where a launch per iteration costs more
+    than the agents; 0 (per-iteration launches) above."""
+    if args.day >= 0:
+        return args.day
+    return 256 if w["n0"] < 1_000_000 else 0
+
+
+def per_iteration_summary(r):
+    """The figures of a single run by per-iteration launches (day graphs)."""
+    return {"value": r["value"], "ms_per_step": r["ms_per_step"], "agent_days_per_s": r["agent_days_per_s"],
+            "roofline_frac": r["roofline"]["frac"], "mean_pass_ms": r["roofline"]["mean_launch_ms"]}
 
 
 def main():
@@ -383,6 +407,9 @@ def main():
                          "4096-agent chunks would leave most work-groups idle)")
     ap.add_argument("--target-chunks", type=int, default=0,
                     help="single run: most chunks per iteration (0: two per resident work-group)")
+    ap.add_argument("--day", type=int, default=-1,
+                    help="single runs: work-groups of the cooperative day launches (0: per-iteration launches; "
+                         "default: 256 below 1M initial agents, else 0)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-days", type=int, default=3, help="CPU baseline: days simulated")
     ap.add_argument("--cpu-scale", type=int, default=0,
@@ -408,7 +435,11 @@ def main():
     if args.warmup + args.steps > w["days"]:
         sys.exit("warmup + steps exceeds the workload's horizon")
     chunk = args.chunk or (512 if w["n0"] < 1_000_000 else 4096)
-    res = single_run(w, args, args.workload, args.steps, args.warmup, chunk)
+    day = day_groups(args, w)
+    res = single_run(w, args, args.workload, args.steps, args.warmup, chunk, day)
+    if day:  # the same run by per-iteration launches, for comparison
+        res["per_iteration_launches"] = per_iteration_summary(single_run(w, args, args.workload, args.steps,
+                                                                         args.warmup, chunk, 0))
     log(f"{args.workload}: {res['value']:.3e} agent-steps/s, frac {res['roofline']['frac']:.3f}")
     copy_bytes = 1 << 30
     copy_ms = S.stream_copy_ms(copy_bytes)
@@ -427,7 +458,10 @@ def main():
                               "kernels": "k_count, k_scan_local, k_scan_top, k_rank"}
     ok = True
     if args.workload == "large" and not args.no_extras:
-        small = single_run(WORKLOADS["small"], args, "small", 20, 2, 512)
+        sw = WORKLOADS["small"]
+        small = single_run(sw, args, "small", 20, 2, 512, day_groups(args, sw))
+        if small["config"]["day_launch_work_groups"]:
+            small["per_iteration_launches"] = per_iteration_summary(single_run(sw, args, "small", 20, 2, 512, 0))
         log(f"small: {small['value']:.3e} agent-steps/s, frac {small['roofline']['frac']:.3f}")
         out["small_run"] = small
         out["batched"], ok_b = batched_measure(args, "sweep", log)

@@ -243,6 +243,12 @@ int sampo_engine_capture(sampo_engine *e, uint32_t n);
 int sampo_engine_replay(sampo_engine *e);
 int sampo_engine_replay_times(sampo_engine *e, float *wall_ms, float *pass_ms);
 
+/* Advance `days` days (sampo_engine_step_days) with a HIP event pair around
+ * each day's launch -- the cooperative day launch, or the day graph -- and
+ * return the time from the first day's start to the last day's end and the
+ * summed per-day durations (synchronises). */
+int sampo_engine_time_days(sampo_engine *e, uint32_t days, float *wall_ms, float *launch_ms);
+
 /* Free and total device memory of HIP device `device`, for admission by footprint. */
 int sampo_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
 

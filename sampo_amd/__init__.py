@@ -114,6 +114,7 @@ def lib():
         L.sampo_engine_histogram.argtypes = [vp, C.c_uint32, vp]
         L.sampo_engine_agent_steps.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int]
         L.sampo_engine_time_pass.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.sampo_engine_time_days.argtypes = [vp, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.sampo_engine_capture.argtypes = [vp, C.c_uint32]
         L.sampo_engine_replay.argtypes = [vp]
         L.sampo_engine_replay_times.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -394,6 +395,13 @@ class Engine:
         """(first pass start to last pass end ms, summed pass ms) of the replay."""
         w, p = C.c_float(), C.c_float()
         _check(lib().sampo_engine_replay_times(self.h, C.byref(w), C.byref(p)))
+        return w.value, p.value
+
+    def time_days(self, days):
+        """(first day start to last day end ms, summed per-day launch ms) over
+        `days` days, an event pair around each day's launch."""
+        w, p = C.c_float(), C.c_float()
+        _check(lib().sampo_engine_time_days(self.h, days, C.byref(w), C.byref(p)))
         return w.value, p.value
 
     def time_pass(self, n):

@@ -655,25 +655,26 @@ __device__ __attribute__((noinline)) static Cut chunk_cut(const Inst *I, uint32_
  * the totals land in tot (shared).  Waves scan with DPP and post their totals
  * to LDS; after one barrier, lane i < N of every wave adds up column i of
  * the waves before its own (its offset) and of all waves (the total), and
- * the offsets reach the wave's lanes by readlane.  Column i of a wave is
- * scanned only if bit i of `cols` (uniform over the wave) is set: the other
- * columns hold zeros in that wave (with few chunks most waves hold none, and
- * the scans are the work-group's VALU time). */
+ * the offsets reach the wave's lanes by readlane.  Every column of every
+ * wave is scanned, without branches, so the N DPP chains interleave (a
+ * branch or a store between two would end each in a block of its own);
+ * columns a wave holds no values in are zeros and scan to zeros. */
 template <int N, int NTH = PREP_WG>
-__device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot, uint32_t cols)
+__device__ static inline void block_excl_n(const uint32_t (&v)[N], uint32_t (&ex)[N], uint32_t *tot)
 {
 	constexpr uint32_t NW = NTH / 64u;
 	static_assert(N <= 64, "one lane per column");
 	__shared__ uint32_t lds[NW][N];
 	const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-	cols = __builtin_amdgcn_readfirstlane(cols);
+	uint32_t x[N];
 #pragma unroll
-	for (int i = 0; i < N; ++i) { /* exclusive within the wave; lane 63 posts the wave total */
-		uint32_t x = 0u;
-		if ((cols >> i) & 1u) x = wave_incl_scan(v[i]);
-		if (lane == 63u) lds[w][i] = x;
-		ex[i] = (cols >> i) & 1u ? x - v[i] : 0u;
+	for (int i = 0; i < N; ++i) x[i] = wave_incl_scan(v[i]);
+	if (lane == 63u) { /* the wave totals */
+#pragma unroll
+		for (int i = 0; i < N; ++i) lds[w][i] = x[i];
 	}
+#pragma unroll
+	for (int i = 0; i < N; ++i) ex[i] = x[i] - v[i];
 	lds_barrier();
 	uint32_t off = 0u, all = 0u;
 	if (lane < (uint32_t)N) {
@@ -708,11 +709,15 @@ __device__ static inline uint32_t run_row(uint32_t sn, uint32_t kind)
 __device__ static uint32_t fold_totals(Inst *I, const IterState &ip, IterState &np, const uint32_t *tot,
                                        uint32_t *kst, uint32_t *vst, uint32_t *sb)
 {
-	const uint32_t *sum = tot + 16; /* newEggs, bites, ..., potentially infective */
+	uint32_t t[24]; /* all reads first: the writes below may alias tot as far as the compiler knows */
+#pragma unroll
+	for (uint32_t i = 0; i < 24u; ++i) t[i] = tot[i];
+	const uint32_t *sum = t + 16; /* newEggs, bites, ..., potentially infective */
 	const uint32_t E = sum[0];
 	uint32_t k = 0, vend = 0, b = E > 0u ? 1u : 0u;
+#pragma unroll
 	for (uint32_t s = 0; s < 8u; ++s) {
-		const uint32_t c = tot[s] + (s == 0u ? E : 0u);
+		const uint32_t c = t[s] + (s == 0u ? E : 0u);
 		np.cnt[s] = c;
 		kst[s] = k;
 		k += c;
@@ -721,7 +726,7 @@ __device__ static uint32_t fold_totals(Inst *I, const IterState &ip, IterState &
 		vst[s] = vs;
 		vend = vs + c;
 		sb[s] = b;
-		b += tot[8 + s];
+		b += t[8 + s];
 	}
 	kst[8] = k;
 	np.nseg = b;
@@ -752,9 +757,8 @@ __device__ static uint32_t fold_totals(Inst *I, const IterState &ip, IterState &
  * With at most 512 chunks the work-group has threads to spare: TPC = 2, 4 or
  * 8 parts of CW threads (whole waves) each take every chunk, part p only the
  * new states s' with s' % TPC == p (and part 0 the sums).  A state's runs of
- * a chunk are still all in one thread, so the scan's offsets are unchanged;
- * each thread writes a TPC-th of the segments, and a wave scans only its
- * parts' columns. */
+ * a chunk are still all in one thread, so the scan's offsets are unchanged,
+ * and each thread writes a TPC-th of the segments. */
 __device__ static void fold_iteration(Inst *I, uint32_t par)
 {
 	constexpr uint32_t KR = 2; /* chunks per thread held in registers */
@@ -785,14 +789,6 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 		for (uint32_t sn = p; sn < 8u; sn += TPC) own |= 1u << sn;
 	}
 	const bool sums = TPC == 1u || tid < CW; /* part 0 (or every thread) adds the chunk sums */
-	/* the columns this wave can hold: counts and runs of its parts' states,
-	 * the sums in part 0; none in a wave without chunks */
-	uint32_t cols = 0u;
-	{
-		const uint32_t t0 = tid & ~63u; /* the wave's first thread */
-		const bool live = TPC == 1u ? t0 * K < C : (t0 / CW < TPC && t0 % CW < C);
-		if (live) cols = own | (own << 8) | (sums ? 0xff0000u : 0u);
-	}
 	const bool in_regs = K <= KR;
 	auto count = [&](uint32_t row, uint32_t c) -> uint32_t { return co[row * cmax + c]; };
 	/* loads: the run counts of this thread's chunks (when few) and their sums.
@@ -854,7 +850,7 @@ __device__ static void fold_iteration(Inst *I, uint32_t par)
 				}
 	}
 	uint32_t ex[24];
-	block_excl_n<24>(wv, ex, s_tot, cols);
+	block_excl_n<24>(wv, ex, s_tot);
 	const uint32_t E = s_tot[16]; /* newEggs */
 	if (tid == 0) {
 		const uint32_t k = fold_totals(I, ip, np, s_tot, s_kst, s_vst, s_sb);
@@ -988,48 +984,66 @@ __device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bi
 			ip.nbseed[q - 16u] = bin[BIN_NBSEED + (q - 16u)];
 	}
 	if (tid == 0) {
-		uint32_t n = 0;
+		/* every read first, then the arithmetic, then every write: the
+		 * compiler cannot tell the record's fields apart through these
+		 * pointers, so a read after a write would wait for the write */
+		uint32_t cnt[8];
+		for (int s = 0; s < 8; ++s) cnt[s] = ip.cnt[s];
+		const uint32_t v0 = ip.vstart[0], v3 = ip.vstart[3], v7 = ip.vstart[7], step = ip.step;
+		const uint32_t l1de = I->l1de, fem = I->females, pinf = I->potinf;
+		const uint32_t ncyc = I->ncyc, scyc = I->scyc, bites = I->bites, ibites = I->ibites;
+		const uint32_t horizon = I->horizon, P = I->P, P_max = I->P_max, mode = I->mode;
+		const uint64_t max_agents = I->max_agents;
+		const float hs0 = I->hmr_stage[0], hs2 = I->hmr_stage[2];
+		sampo_row *const rows = I->rows;
+		uint32_t bw[BIN_SEED];
+		for (uint32_t q = 0; q < BIN_SEED; ++q) bw[q] = bin[q];
+		uint32_t kst[9], n = 0;
 		for (int s = 0; s < 8; ++s) {
-			ip.kstart[s] = n;
-			n += ip.cnt[s];
+			kst[s] = n;
+			n += cnt[s];
 		}
-		ip.kstart[8] = n;
-		ip.n = n;
+		kst[8] = n;
 		sampo_row row;
-		for (int s = 0; s < 8; ++s) row.count[s] = ip.cnt[s];
-		uint32_t adults_range = (ip.vstart[7] + ip.cnt[7]) - ip.vstart[3];
-		row.l1de = ip.cnt[1] > 0 ? I->l1de : 0u;
-		row.females = adults_range > 0 ? I->females : 0u;
-		row.potInfective = adults_range > 0 ? I->potinf : 0u;
-		row.numCycles = I->ncyc;
-		row.sumCycles = I->scyc;
-		row.bites = I->bites;
-		row.infectBites = I->ibites;
+		for (int s = 0; s < 8; ++s) row.count[s] = cnt[s];
+		uint32_t adults_range = (v7 + cnt[7]) - v3;
+		row.l1de = cnt[1] > 0 ? l1de : 0u;
+		row.females = adults_range > 0 ? fem : 0u;
+		row.potInfective = adults_range > 0 ? pinf : 0u;
+		row.numCycles = ncyc;
+		row.sumCycles = scyc;
+		row.bites = bites;
+		row.infectBites = ibites;
 		row.valid = 1u;
 		uint32_t st;
-		if (ip.step >= I->horizon) {
+		if (step >= horizon) {
 			st = SAMPO_DONE;
-		} else if (!((double)n <= (double)I->max_agents * 0.95)) {
+		} else if (!((double)n <= (double)max_agents * 0.95)) {
 			st = SAMPO_CAPACITY;
 			row.valid = 0u;
 		} else if (n == 0u) {
 			st = SAMPO_EXTINCT;
 			row.valid = 0u;
-		} else if (n > I->P) { /* the output would not fit: grow, or give up */
-			st = I->P < I->P_max ? SAMPO_GROW : SAMPO_PHYS_FULL;
+		} else if (n > P) { /* the output would not fit: grow, or give up */
+			st = P < P_max ? SAMPO_GROW : SAMPO_PHYS_FULL;
 		} else {
 			st = SAMPO_RUNNING;
 		}
-		if (write_row) gput(&I->rows[ip.step], row);
-		ip.temp = sc_float(bin[BIN_TEMP]);
+		const float temp = sc_float(bw[BIN_TEMP]);
 		SpRow sr;
-		for (uint32_t q = 0; q < sizeof(SpRow) / 4u; ++q) ((uint32_t *)&sr)[q] = bin[BIN_SR + q];
-		ip.nb_inc = sc_float(bin[BIN_INC]);
+		for (uint32_t q = 0; q < sizeof(SpRow) / 4u; ++q) ((uint32_t *)&sr)[q] = bw[BIN_SR + q];
+		double csd_inc;
 		{
-			float q = 111.0f / (ip.temp - 16.0f);
-			ip.csd_inc = 1.0 / ((double)q * 14.0);
+			float q = 111.0f / (temp - 16.0f);
+			csd_inc = 1.0 / ((double)q * 14.0);
 		}
-		for (uint32_t q = 0; q < sizeof(sampo_env) / 4u; ++q) ((uint32_t *)&ip.env)[q] = bin[BIN_ENV + q];
+		for (int s = 0; s < 9; ++s) ip.kstart[s] = kst[s];
+		ip.n = n;
+		if (write_row) gput(&rows[step], row);
+		ip.temp = temp;
+		ip.nb_inc = sc_float(bw[BIN_INC]);
+		ip.csd_inc = csd_inc;
+		for (uint32_t q = 0; q < sizeof(sampo_env) / 4u; ++q) ((uint32_t *)&ip.env)[q] = bw[BIN_ENV + q];
 		ip.open = sr.open;
 		ip.dev_rate = sr.dev_rate;
 		ip.pupa_dly = sr.pupa_dly;
@@ -1037,13 +1051,13 @@ __device__ static void begin_iteration(Inst *I, uint32_t par, const uint32_t *bi
 		ip.bmd_dly = sr.bmd_dly;
 		ip.itn_kill = sr.itn_kill;
 		ip.irs_kill = sr.irs_kill;
-		ip.nb_set = ip.step;
-		ip.seed_set = ip.step + 1u;
-		ip.h = sc_f2u((float)ip.step * 1.0f) % 24u;
-		ip.tb = I->l1de + (ip.vstart[0] + ip.cnt[0]) + ip.cnt[2]; /* killAgents.cl:42 */
-		if (I->mode == SAMPO_MODE_CONFIG) {
-			ip.hmr_egg = I->hmr_stage[0];
-			ip.hmr_pupa = I->hmr_stage[2];
+		ip.nb_set = step;
+		ip.seed_set = step + 1u;
+		ip.h = sc_f2u((float)step * 1.0f) % 24u;
+		ip.tb = l1de + (v0 + cnt[0]) + cnt[2]; /* killAgents.cl:42 */
+		if (mode == SAMPO_MODE_CONFIG) {
+			ip.hmr_egg = hs0;
+			ip.hmr_pupa = hs2;
 		} else {
 			ip.hmr_egg = sm_hmr(0.1f);
 			ip.hmr_pupa = sm_hmr(0.1f);
@@ -1119,10 +1133,12 @@ __device__ static inline uint32_t seg_find_wave(const uint32_t *k0, uint32_t lo,
  * k0/st/pb/vb (dense start, state | SEG_REV, physical base, virtual base).
  * Thread c < C polls chunk c's NOUT words into registers, all at once and
  * again only those not yet published, and places its runs after one scan
- * over the work-group.  Returns false in every thread if a wait gave up. */
+ * over the work-group.  *seg0 receives the index of the segment holding
+ * the first agent of this work-group's chunk (blockIdx.x) of the next
+ * iteration.  Returns false in every thread if a wait gave up. */
 template <int NTH>
 __device__ static bool day_fold(Inst &sI, uint32_t par, DaySync *sync, uint32_t j, uint32_t tag, uint32_t *sk0,
-                                uint32_t *sst, uint32_t *spb, uint32_t *svb)
+                                uint32_t *sst, uint32_t *spb, uint32_t *svb, uint32_t *seg0)
 {
 	static_assert(NTH >= (int)DAY_CHUNKS, "a thread per chunk");
 	__shared__ uint32_t s_kst[9], s_vst[8], s_sb[8], s_tot[24];
@@ -1175,9 +1191,12 @@ __device__ static bool day_fold(Inst &sI, uint32_t par, DaySync *sync, uint32_t 
 		}
 	}
 	uint32_t ex[24];
-	block_excl_n<24, NTH>(wv, ex, s_tot, (tid & ~63u) < C ? 0xffffffu : 0u);
+	block_excl_n<24, NTH>(wv, ex, s_tot); /* zeros past C */
 	const uint32_t E = s_tot[16];
-	if (tid == 0) (void)fold_totals(&sI, ip, np, s_tot, s_kst, s_vst, s_sb);
+	if (tid == 0) {
+		(void)fold_totals(&sI, ip, np, s_tot, s_kst, s_vst, s_sb);
+		*seg0 = 0u; /* the newborns', unless a run below holds the chunk start */
+	}
 	lds_barrier();
 	if (tid == 0 && E > 0u) { /* the newborns */
 		sk0[0] = 0u;
@@ -1186,12 +1205,24 @@ __device__ static bool day_fold(Inst &sI, uint32_t par, DaySync *sync, uint32_t 
 		svb[0] = 0u;
 	}
 	if (mine) {
+		/* the bases, read before the first write (which may alias them as far
+		 * as the compiler knows) */
+		uint32_t kst[8], vst[8], sbs[8], okst[9];
+#pragma unroll
+		for (uint32_t q = 0; q < 8u; ++q) {
+			kst[q] = s_kst[q];
+			vst[q] = s_vst[q];
+			sbs[q] = s_sb[q];
+		}
+#pragma unroll
+		for (uint32_t q = 0; q < 9u; ++q) okst[q] = ip.kstart[q];
 		const Cut oc = ip.cut;
-		const uint32_t ck0 = cut_start(oc, tid), ce = cut_start(oc, tid + 1u), ck1 = ce < ip.n ? ce : ip.n;
+		const uint32_t on = ip.n, km = cut_start(np.cut, blockIdx.x);
+		const uint32_t ck0 = cut_start(oc, tid), ce = cut_start(oc, tid + 1u), ck1 = ce < on ? ce : on;
 #pragma unroll
 		for (uint32_t sn = 0; sn < 8u; ++sn) {
-			uint32_t k0 = s_kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
-			uint32_t idx = s_sb[sn] + ex[8u + sn];
+			uint32_t k0 = kst[sn] + (sn == 0u ? E : 0u) + ex[sn];
+			uint32_t idx = sbs[sn] + ex[8u + sn];
 #pragma unroll
 			for (uint32_t kind = 0; kind < 3u; ++kind) {
 				if (kind >= runs_of(sn)) continue;
@@ -1199,12 +1230,13 @@ __device__ static bool day_fold(Inst &sI, uint32_t par, DaySync *sync, uint32_t 
 				if (!n) continue;
 				/* the source block: chunk tid's part of the state-src range */
 				const uint32_t src = row & 7u, mv = row >> 3;
-				const uint32_t bh = ip.kstart[src] > ck0 ? ip.kstart[src] : ck0;
-				const uint32_t be = ip.kstart[src + 1u] < ck1 ? ip.kstart[src + 1u] : ck1;
+				const uint32_t bh = okst[src] > ck0 ? okst[src] : ck0;
+				const uint32_t be = okst[src + 1u] < ck1 ? okst[src + 1u] : ck1;
 				sk0[idx] = k0;
 				sst[idx] = sn | (mv ? SEG_REV : 0u);
 				spb[idx] = mv ? be - 1u : bh;
-				svb[idx] = s_vst[sn] - s_kst[sn] + k0;
+				svb[idx] = vst[sn] - kst[sn] + k0;
+				if (km >= k0 && km - k0 < n) *seg0 = idx;
 				++idx;
 				k0 += n;
 			}
@@ -1472,6 +1504,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	struct DayLds {
 		Inst sI;
 		uint32_t bin[DAY_ITERS][BIN_WORDS];
+		uint32_t seg0; /* the segment holding this work-group's chunk start, from the fold (~0u: search) */
 	};
 	__shared__ std::conditional_t<DAY, DayLds, uint32_t> s_day;
 	uint32_t par = 0, iters = 0, j = 0; /* day launch: parity, iterations to run, the current one */
@@ -1530,6 +1563,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				s_pb[i] = sg[4u * i + 2u];
 				s_vb[i] = sg[4u * i + 3u];
 			}
+			if (tid == 0) s_day.seg0 = ~0u; /* no fold yet: the chunk's first segment is searched for */
 			const uint32_t step0 = sI.st[par].step;
 			for (uint32_t x = tid; x < iters * BIN_WORDS; x += WG) {
 				const uint32_t jj = x / BIN_WORDS;
@@ -1544,7 +1578,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			if (nt >= T.ip.nchunk) { /* the iteration's chunks are done: fold it, begin the next */
 				Inst &sI = s_day.sI;
 				if (j + 1u == iters && blockIdx.x != 0u) return; /* work-group 0 alone folds the last */
-				if (!day_fold<WG>(sI, par, da.sync, j, da.seq0 + j, s_k0, s_st, s_pb, s_vb)) {
+				if (!day_fold<WG>(sI, par, da.sync, j, da.seq0 + j, s_k0, s_st, s_pb, s_vb, &s_day.seg0)) {
 					fault = true;
 					break;
 				}
@@ -1634,9 +1668,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 			}
 		}
 		if constexpr (DAY) {
-			if (w == 0) { /* the chunk's range from the iteration's cut; its first segment from the window */
+			if (w == 0) { /* the chunk's range from the iteration's cut; its first segment from the fold or the window */
 				const uint32_t e = cut_start(T.ip.cut, c + 1u), ck0 = cut_start(T.ip.cut, c);
-				const uint32_t g = seg_find_wave(s_k0, 0u, s_wm, ck0);
+				const uint32_t pre = s_day.seg0;
+				const uint32_t g = c == blockIdx.x && pre != ~0u ? pre : seg_find_wave(s_k0, 0u, s_wm, ck0);
 				if (lane == 0) {
 					T.ch.k0 = ck0;
 					T.ch.k1 = e < T.ip.n ? e : T.ip.n;
@@ -3066,6 +3101,35 @@ extern "C" int sampo_engine_replay_times(sampo_engine *e, float *wall_ms, float 
 	}
 	HIPCHK(hipEventElapsedTime(wall_ms, e->cap_ev[0], e->cap_ev[2 * n - 1]));
 	*pass_ms = sum;
+	return 0;
+}
+
+extern "C" int sampo_engine_time_days(sampo_engine *e, uint32_t days, float *wall_ms, float *launch_ms)
+{
+	if (!e || !wall_ms || !launch_ms || days == 0) return fail(SAMPO_E_ARG, "bad arguments%s%ld");
+	HIPCHK(hipSetDevice(e->device));
+	{ /* a day graph is captured before the first event */
+		const int rc = sampo_engine_step_days(e, 0);
+		if (rc) return rc;
+	}
+	std::vector<hipEvent_t> ev(2 * (size_t)days);
+	for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+	for (uint32_t d = 0; d < days; ++d) {
+		HIPCHK(hipEventRecord(ev[2 * d], e->stream));
+		const int rc = sampo_engine_step_days(e, 1);
+		if (rc) return rc;
+		HIPCHK(hipEventRecord(ev[2 * d + 1], e->stream));
+	}
+	HIPCHK(hipEventSynchronize(ev[2 * (size_t)days - 1]));
+	float sum = 0.0f;
+	for (uint32_t d = 0; d < days; ++d) {
+		float ms = 0.0f;
+		HIPCHK(hipEventElapsedTime(&ms, ev[2 * d], ev[2 * d + 1]));
+		sum += ms;
+	}
+	HIPCHK(hipEventElapsedTime(wall_ms, ev[0], ev[2 * (size_t)days - 1]));
+	for (auto &x : ev) (void)hipEventDestroy(x);
+	*launch_ms = sum;
 	return 0;
 }
 

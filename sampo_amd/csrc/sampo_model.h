@@ -79,11 +79,12 @@ SC_FN int sm_dev_index(const sampo_species *sp, float t)
 SC_FN float sm_dev_rate(const sampo_species *sp, float t) { return sp->dev_rates[sm_dev_index(sp, t)]; }
 
 /* eggsHatching (properties.h:39-60): the first band whose threshold exceeds
- * p, rounded to an integer number of hours */
+ * p, rounded to an integer number of hours.  The thresholds ascend
+ * (sampo_species_check), so that band's index is the number of thresholds p
+ * is not below: counted without branches, it costs one round of loads. */
 SC_FN float sm_hatching(const float *thr, const float *m, const float *b, float p)
 {
-	int i = 0;
-	while (i < 4 && !(p < thr[i])) ++i;
+	const int i = (int)!(p < thr[0]) + (int)!(p < thr[1]) + (int)!(p < thr[2]) + (int)!(p < thr[3]);
 	return (float)sc_f2u(m[i] * p + b[i] + 0.5f);
 }
 
