@@ -181,8 +181,8 @@ int sampo_engine_set_grid(sampo_engine *e, uint32_t per_cu);
  * src/abms.c:502-529).  work_groups > 0: such an instance is cut into at
  * most work_groups equal chunks, and while it is the only instance of the
  * engine, sampo_engine_step() and sampo_engine_step_days() advance it by
- * cooperative launches of work_groups work-groups, each running up to 24
- * iterations (a simulated day) with no kernel boundary between them.  For
+ * cooperative launches of work_groups work-groups, each running up to 96
+ * iterations (four simulated days) with no kernel boundary between them.  For
  * instances small enough that a launch per iteration costs more than their
  * agents (about 1M agents and below).  Results do not depend on it.
  * work_groups must be at most 256 and at most the work-groups the device can

@@ -327,8 +327,8 @@ class Engine:
     def set_day(self, work_groups):
         """Day launches for later admissions: such an instance is cut into at
         most work_groups equal chunks and, while it is the engine's only
-        instance, advances by one cooperative launch per day (up to 24
-        iterations each).  0: off."""
+        instance, advances by cooperative launches of up to 96 iterations
+        (four days) each.  0: off."""
         _check(lib().sampo_engine_set_day(self.h, work_groups))
 
     def set_chunk(self, agents, target_chunks=0):

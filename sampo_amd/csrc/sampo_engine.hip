@@ -254,7 +254,9 @@ struct Ctx {
 };
 
 /* Day launches (k_day): one cooperative launch runs up to DAY_ITERS
- * iterations of a lone instance cut into at most DAY_CHUNKS equal chunks.
+ * iterations of a lone instance cut into at most DAY_CHUNKS equal chunks
+ * (one launch per day stepped, up to four days per launch when several are:
+ * successive cooperative launches are ~20 us apart).
  * Each work-group publishes its chunk's NOUT results as 64-bit words tagged
  * with the iteration's sequence number (high half) in the parity of the
  * iteration; every work-group waits for all of them and folds them itself.
@@ -271,7 +273,7 @@ struct Ctx {
  * after every wave's agent stores have completed (s_waitcnt and a barrier),
  * and reads agents only after it has seen every tag of the iteration. */
 #define DAY_CHUNKS 256u
-#define DAY_ITERS 24u
+#define DAY_ITERS 96u /* four days: one launch per four days when several are stepped at once */
 #define DAY_SEGMAX (16u * DAY_CHUNKS + 2u) /* segments of an iteration with <= DAY_CHUNKS chunks */
 #define DAY_POLLS (1u << 20)
 
