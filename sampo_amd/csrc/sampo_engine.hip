@@ -369,8 +369,13 @@ __device__ static inline void ev_add(lds_u32 *ev, uint32_t i, uint32_t v)
 __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, const TileParams &T, lds_u32 *ev)
 {
 	const IterState &ip = T.ip;
-	/* killAgents.cl:55-147 */
-	uint32_t d = sc_f2u(sc_minf(a.age / 24.0f, 99.0f));
+	/* killAgents.cl:55-147.  The age in days, (uint)min(age / 24.0f, 99.0f):
+	 * ages are whole hours below 2^16 (admission checks it), for which the
+	 * correctly rounded quotient truncates to the integer one (checked for
+	 * every such age), so it is one integer division by a constant instead
+	 * of a float division sequence per agent. */
+	const uint32_t dq = (uint32_t)a.age / 24u;
+	const uint32_t d = dq < 99u ? dq : 99u;
 	float hmr = s == S_EGG ? ip.hmr_egg : s == S_LARVA ? ip.hmr_larva[d] : s == S_PUPA ? ip.hmr_pupa
 	                                                                                 : T.hmr_adult[d];
 	if (rnd(&T.ip.seed[0], g) <= hmr) return S_DEAD;
@@ -380,10 +385,18 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	a.age += 1.0f;
 	a.his += 1.0f;
 	uint32_t hbc = a.misc & 0xffffu;
-	{ /* the increment depends on T only; k_prep evaluates it once per iteration */
-		int cond = (t > 16.0f && hbc > 0u);
-		a.csd = (float)((double)a.csd + (double)cond * ip.csd_inc);
-	}
+	/* updateAgents.cl:69-70: csd + cond * (1/((111/(T-16))*14)) in binary64; the
+	 * increment depends on T only (evaluated once per iteration).  T is
+	 * finite, so the increment is finite or a zero, with the sign of T - 16
+	 * (T - 16 is exact and a zero only at 16, where the increment is +0).
+	 * Without the condition the addend 0 * increment is a zero of that sign,
+	 * and adding it in binary64 and rounding back is adding the same zero in
+	 * binary32, bit for bit (-0 + -0 = -0, -0 + +0 = +0, NaN quieted), so
+	 * the binary64 work is only done for the agents that have bitten. */
+	if (t > 16.0f && hbc > 0u)
+		a.csd = (float)((double)a.csd + ip.csd_inc);
+	else
+		a.csd = a.csd + __uint_as_float(__float_as_uint(t - 16.0f) & 0x80000000u);
 	const uint32_t open = ip.open;
 	switch (s) {
 	case S_EGG: /* :72-87 */
@@ -1780,7 +1793,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 						pack_agent(x, wv);
 #pragma unroll
 						for (int f = 0; f < NFIELD; ++f) cur[f][o] = wv[f];
-						if (ns2 == S_LARVA) l1de += sc_f2u(x.age / 24.0f);
+						if (ns2 == S_LARVA) l1de += (uint32_t)x.age / 24u; /* calcL1de.cl: (uint)(age / 24.0f), whole hours */
 						if (ns2 >= S_IMM && (x.misc >> 31)) {
 							fem += 1u;
 							pinf += x.csd >= 1.0f ? 1u : 0u;

@@ -23,9 +23,11 @@ def rows_equal(orows, grows):
 
 
 def agents_equal(a, b):
+    """Field by field, as bits (a -0 is not a +0, a NaN equals itself)."""
     assert a.shape == b.shape
     for f in a.dtype.names:
-        assert np.array_equal(a[f], b[f]), f
+        u = f"u{a[f].dtype.itemsize}"
+        assert np.array_equal(np.ascontiguousarray(a[f]).view(u), np.ascontiguousarray(b[f]).view(u)), f
 
 
 def ref_inputs(steps, cc, temp=30.0):
@@ -111,6 +113,38 @@ def test_engine_full_record_when_immatures_carry_adult_words(field):
         lay_g, ag_g = e.export(slot)
         assert np.array_equal(lay_o, lay_g)
         agents_equal(ag_o, ag_g)
+
+
+def test_engine_development_sums_of_minus_zero_keep_their_sign():
+    """Adults that start with a development sum of -0 keep it while it does
+    not grow: the reference adds cond * increment in binary64, which is -0
+    below 16 degC (the increment is negative there) and +0 above, so an
+    adult's -0 stays -0 on cold hours and turns +0 on warm ones.  The run
+    crosses 16 degC both ways; the exported sums are compared as bits."""
+    n0, days, seed, dmr = 4000, 4, 23, (0.04, 0.03, 0.02, 0.1)
+    steps = 24 * days
+    temp = (15.5 - 3.0 * np.sin(np.arange(steps) * (2.0 * np.pi / 24.0))).astype(np.float32)
+    assert (temp < 16.0).any() and (temp > 16.0).any()
+    lay, ag = S.gen_population(seed, n0, float(temp[0]))
+    n_imm = sum(int(lay[s][1] - lay[s][0]) for s in range(3))
+    ag = ag.copy()
+    ag["csd"][n_imm:] = np.float32(-0.0)
+    env = S.make_env(cc=5.0 * n0)
+    sim = O.Sim(temp=temp, env=env, capacity=60 * n0, seed=seed, layout=lay, agents=ag, mode=1, dmr=dmr)
+    with S.Engine(max_slots=1) as e:
+        slot = e.admit(mode=S.MODE_CONFIG, seed=seed, horizon=steps, temp=temp, env=env, max_agents=60 * n0,
+                       dmr=dmr, layout=lay, agents=ag)
+        signs, orows = [], []
+        for k in range(steps):
+            orows.append(sim.run(1))
+            e.step(1)
+            lay_o, ag_o, _ = sim.export()
+            lay_g, ag_g = e.export(slot)
+            assert np.array_equal(lay_o, lay_g)
+            agents_equal(ag_o, ag_g)
+            signs.append(int(np.count_nonzero(np.signbit(ag_g["csd"]))))
+        rows_equal(np.concatenate(orows), e.rows(slot, 0, steps))
+    assert max(signs) > 0 and min(signs) < max(signs)  # some -0 sums kept, some turned
 
 
 def test_engine_refuses_non_finite_temperatures():
