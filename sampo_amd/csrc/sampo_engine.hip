@@ -200,8 +200,9 @@ struct IterState {
 	float ms_dly, bms_dly, larva_mean, larva_sd, batch_mean, batch_sd, batch_decay, lay_p, bite_human;
 	float hatch_thr[4], hatch_m[5], hatch_b[5];
 	/* the iteration's 16 seeds (set seed_set) and seeds[2], seeds[3] of the
-	 * set that laid the newborns (nb_set), copied here by the prep so that
-	 * the pass reads them with the rest of this record */
+	 * set that laid the newborns (nb_set), four prepared sets of four
+	 * (sm_taus_prepare) and two, copied here by the prep so that the pass
+	 * reads them with the rest of this record */
 	uint32_t seed[16], nbseed[8];
 	double csd_inc;        /* 1/((111/(T-16))*14) of updateAgents.cl:69-70, in binary64 */
 	float hmr_larva[100];
@@ -222,7 +223,7 @@ struct Inst {
 	Seg *seg[2];           /* iteration with parity q reads seg[q] ... */
 	Chunk *chunk[2];       /* ... and chunk[q] */
 	uint32_t *cout;
-	const uint32_t *seeds; /* (horizon + 1) sets of 16 values */
+	const uint32_t *seeds; /* (horizon + 1) x 4 sets of 4 seeds, each prepared by sm_taus_prepare */
 	const float *temp;
 	const sampo_env *env;
 	sampo_row *rows;       /* horizon + 1 */
@@ -333,9 +334,10 @@ struct TileParams {
 	uint32_t P, nseg, cmax, compact;
 };
 
+/* HybridTaus of virtual index g with a prepared seed set (sm_taus_prepare) */
 __device__ static inline float rnd(const uint32_t *z, uint32_t g)
 {
-	return sc_hybrid_taus(z[0], z[1], z[2], z[3], g);
+	return sm_taus_draw(z, g);
 }
 
 /* The transcendental paths run only on state transitions; keeping them out
@@ -509,7 +511,8 @@ __device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uin
 	unpack_agent(w, a, fields);
 }
 
-/* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3);
+/* initAgents.cl:34-64 for newborn egg g, laid with seeds z (sets 2 and 3,
+ * prepared by sm_taus_prepare);
  * inc is eggsIncubation at the laying temperature, thr/m/b the eggsHatching
  * bands */
 __device__ static inline void newborn(Ag &a, const uint32_t *z, float inc, const float *thr, const float *m,
@@ -521,8 +524,8 @@ __device__ static inline void newborn(Ag &a, const uint32_t *z, float inc, const
 	a.aux = 0u;
 	a.avail = 0u;
 	a.cyc = 0u;
-	float u3 = sc_hybrid_taus(z[4], z[5], z[6], z[7], g);
-	float u2 = sc_hybrid_taus(z[0], z[1], z[2], z[3], g);
+	float u3 = sm_taus_draw(z + 4, g);
+	float u2 = sm_taus_draw(z, g);
 	a.misc = ((double)u3 <= 0.5) ? 0x80000000u : 0u;
 	a.dly = inc + sm_hatching(thr, m, b, u2);
 }
@@ -2532,6 +2535,10 @@ extern "C" int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uin
 	const size_t seeds_n = (size_t)(sc->horizon + 1u) * 16u;
 	std::vector<uint32_t> seeds(seeds_n);
 	sampo_seed_stream(sc->seed, (uint32_t)seeds_n, seeds.data());
+	for (size_t q = 0; q < seeds_n; q += 4) {
+		uint32_t z[4] = {seeds[q], seeds[q + 1], seeds[q + 2], seeds[q + 3]};
+		sm_taus_prepare(z, &seeds[q]);
+	}
 	h.buf[0] = (uint32_t *)S.agents;
 	h.buf[1] = h.buf[0] + (size_t)NFIELD * P;
 	h.seg[0] = (Seg *)(base + L.o_seg0);

@@ -133,4 +133,30 @@ SC_FN float sm_hmr_larva(uint32_t d, uint32_t l1de, float cc)
 
 SC_FN uint32_t sm_roundup(uint32_t v, uint32_t m) { return ((v + m - 1u) / m) * m; }
 
+/* HybridTaus (sc_hybrid_taus_bits, include/gpuRand.h:13-50) with the work
+ * that depends on the seeds alone done once per set.  Of a set (z1, z2, z3,
+ * z4), the first and third Tausworthe components do not involve the
+ * work-item index gid, the second steps z2 + 128 + gid, and the LCG term is
+ * 1664525 * (z4 * gid) + 1013904223 = (1664525 * z4) * gid + 1013904223
+ * modulo 2^32.  A prepared set holds {the first and third components xored,
+ * z2 + 128, 1664525 * z4, 0}; a draw then costs one Tausworthe step, one
+ * multiply-add and two xors instead of three steps and two multiplies. */
+SC_FN void sm_taus_prepare(const uint32_t z[4], uint32_t out[4])
+{
+	const uint32_t k = sc_taus_step(z[0] + 128u, 13, 19, 12, 0xfffffffeu) ^
+	                   sc_taus_step(z[2] + 128u, 3, 11, 17, 0xfffffff0u);
+	const uint32_t b = z[1] + 128u, m = 1664525u * z[3];
+	out[0] = k;
+	out[1] = b;
+	out[2] = m;
+	out[3] = 0u;
+}
+
+/* sc_hybrid_taus(z1, z2, z3, z4, gid) of the set p prepared by sm_taus_prepare */
+SC_FN float sm_taus_draw(const uint32_t *p, uint32_t gid)
+{
+	const uint32_t bits = p[0] ^ sc_taus_step(p[1] + gid, 2, 25, 4, 0xfffffff8u) ^ (p[2] * gid + 1013904223u);
+	return sc_float(0x2f800000u) * (float)bits;
+}
+
 #endif

@@ -8,6 +8,8 @@ for the device's (both round to nearest even and keep denormals).
 """
 import numpy as np
 
+from oracle import pyoracle as O
+
 F32 = np.float32
 
 
@@ -67,3 +69,31 @@ def test_adding_a_zero_in_binary64_is_adding_it_in_binary32():
             wide = (x.astype(np.float64) + z).astype(F32)
             narrow = x + F32(z)
         assert np.array_equal(wide.view(np.uint32), narrow.view(np.uint32))
+
+
+def taus_step(z, s1, s2, s3, m):
+    b = ((z << np.uint32(s1)) ^ z) >> np.uint32(s2)
+    return ((z & np.uint32(m)) << np.uint32(s3)) ^ b
+
+
+def test_prepared_seed_sets_draw_what_hybrid_taus_draws():
+    """The engine prepares each seed set once (the first and third Tausworthe
+    components xored, z2 + 128, 1664525 * z4) and draws from it with one
+    step, one multiply-add and two xors; the draws equal the oracle's
+    HybridTaus on random seed sets and indices, the extreme ones included."""
+    rng = np.random.default_rng(4)
+    n = 4096
+    z = rng.integers(0, 2**32, (n, 4), dtype=np.uint64).astype(np.uint32)
+    z[:8] = [[0, 0, 0, 0], [2**32 - 1] * 4, [2**32 - 128] * 4, [127, 2**32 - 129, 128, 1],
+             [1, 2, 3, 4], [2**31, 2**31, 2**31, 2**31], [5, 2**32 - 128, 7, 2**31 - 1], [9, 9, 9, 9]]
+    gid = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    gid[:8] = [0, 1, 2**31, 2**32 - 1, 64, 2**32 - 128, 12345, 2**31 - 1]
+    with np.errstate(over="ignore"):
+        k = taus_step(z[:, 0] + np.uint32(128), 13, 19, 12, 0xfffffffe) ^ \
+            taus_step(z[:, 2] + np.uint32(128), 3, 11, 17, 0xfffffff0)
+        b = z[:, 1] + np.uint32(128)
+        m = np.uint32(1664525) * z[:, 3]
+        bits = k ^ taus_step(b + gid, 2, 25, 4, 0xfffffff8) ^ (m * gid + np.uint32(1013904223))
+    draws = F32(2.0**-32) * bits.astype(F32)
+    want = np.array([O.hybrid_taus(z[i], int(gid[i])) for i in range(n)], F32)
+    assert np.array_equal(draws.view(np.uint32), want.view(np.uint32))
