@@ -1390,7 +1390,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	__shared__ uint32_t s_wlo, s_wm, s_wend; /* window: global index, size, dense end */
 	__shared__ uint32_t s_wc[APT][NW][16];    /* per (input state, moved) */
 	__shared__ uint32_t s_run[16], s_tc[16];
-	__shared__ uint32_t s_head[8], s_end[8];  /* the chunk's block of each input state */
+	__shared__ uint32_t s_base[16];  /* per class: the first slot of its end of the chunk's block of its input state */
 	__shared__ uint32_t s_acc[NW][3];   /* l1de, females, potentially infective: per wave */
 	__shared__ uint32_t s_ev[NW][NEV]; /* agent_step's event sums of the chunk, per wave */
 	__shared__ uint32_t bufA[NFIELD][TILE], bufB[NFIELD][TILE];
@@ -1708,8 +1708,9 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 		if (tid < 8u) {
 			uint32_t h = T.ip.kstart[tid] > T.ch.k0 ? T.ip.kstart[tid] : T.ch.k0;
 			uint32_t z = T.ip.kstart[tid + 1u] < T.ch.k1 ? T.ip.kstart[tid + 1u] : T.ch.k1;
-			s_head[tid] = h;
-			s_end[tid] = z > h ? z : h;
+			/* stayers fill the block from its head up, movers from its end down */
+			s_base[2u * tid] = h;
+			s_base[2u * tid + 1u] = (z > h ? z : h) - 1u;
 		}
 		const uint32_t k0 = T.ch.k0, k1 = T.ch.k1;
 		lds_u32 *const ev = (lds_u32 *)&s_ev[w][0]; /* this wave's event sums */
@@ -1833,20 +1834,33 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				if (cl < 16u) {
 					const uint32_t o = (uint32_t)jx * WG + tid;
 					const uint32_t off = s_wc[jx][w][cl] + lr[jx];
-					const uint32_t p = (cl & 1u) ? s_end[cl >> 1] - 1u - off : s_head[cl >> 1] + off;
-					/* the words the agent's new state keeps: a mover from state s is now in
-					 * s + 1 (gravid: blood-meal seeking, which keeps all) */
-					const uint32_t sn = (cl >> 1) + (cl & 1u);
-					const uint32_t fm = sm_fields(sn > S_GRAVID ? S_BMS : sn, T.compact);
+					const uint32_t p = s_base[cl] + ((cl & 1u) ? 0u - off : off);
+					uint32_t wv[NFIELD];
 #pragma unroll
-					for (int f = 0; f < NFIELD; ++f)
-						if ((fm >> f) & 1u) {
-							if constexpr (DAY) /* coherent at agent scope: other work-groups read it next */
-								__hip_atomic_store((uint32_t *)at_slot(fout[f], p), cur[f][o], __ATOMIC_RELAXED,
-								                   __HIP_MEMORY_SCOPE_AGENT);
-							else
-								*at_slot(fout[f], p) = cur[f][o];
-						}
+					for (int f = 0; f < NFIELD; ++f) wv[f] = cur[f][o]; /* all read at once: one wait */
+					auto put = [&](int f) {
+						if constexpr (DAY) /* coherent at agent scope: other work-groups read it next */
+							__hip_atomic_store((uint32_t *)at_slot(fout[f], p), wv[f], __ATOMIC_RELAXED,
+							                   __HIP_MEMORY_SCOPE_AGENT);
+						else
+							*at_slot(fout[f], p) = wv[f];
+					};
+					/* the words the agent's new state keeps (sm_fields): a mover from
+					 * state s is now in s + 1 (gravid: blood-meal seeking, an adult).
+					 * Every state keeps AH, DLY and NF; AUX from the larva on; CSD,
+					 * AVAIL and CH in the adult states, or every word when the
+					 * instance stores the full record. */
+					const uint32_t sn = (cl >> 1) + (cl & 1u);
+					const bool full = !T.compact;
+					put(F_AH);
+					put(F_DLY);
+					put(F_NF);
+					if (full || sn != S_EGG) put(F_AUX);
+					if (full || sn > S_PUPA) {
+						put(F_CSD);
+						put(F_AVAIL);
+						put(F_CH);
+					}
 				}
 			}
 			/* the next tile rewrites s_wc only after its first barrier */
