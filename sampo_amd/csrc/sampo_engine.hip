@@ -286,14 +286,15 @@ struct DaySync {
 /* ------------------------------------------------------------ the agent */
 
 struct Ag {
-	float age, his, dly, csd;
+	uint32_t age, his;  /* ageInHours, hoursInState: whole hours below 2^16, as integers */
+	float dly, csd;
 	uint32_t aux, avail, cyc, misc; /* misc: humanBloodmealCount (0-15), numEggBatches (16-30), isFemale (31) */
 };
 
 /* the stored words of an agent (sampo_model.h) and back */
 __host__ __device__ static inline void pack_agent(const Ag &a, uint32_t (&w)[NFIELD])
 {
-	w[F_AH] = (uint32_t)a.age | ((uint32_t)a.his << 16);
+	w[F_AH] = a.age | (a.his << 16);
 	w[F_DLY] = sc_bits_f(a.dly);
 	w[F_AUX] = a.aux;
 	w[F_CSD] = sc_bits_f(a.csd);
@@ -307,8 +308,8 @@ __host__ __device__ static inline void unpack_agent(const uint32_t (&w0)[NFIELD]
 	uint32_t w[NFIELD];
 #pragma unroll
 	for (int f = 0; f < NFIELD; ++f) w[f] = (fields >> f) & 1u ? w0[f] : 0u; /* words not stored are zero */
-	a.age = (float)(w[F_AH] & 0xffffu);
-	a.his = (float)(w[F_AH] >> 16);
+	a.age = w[F_AH] & 0xffffu;
+	a.his = w[F_AH] >> 16;
 	a.dly = sc_float(w[F_DLY]);
 	a.aux = w[F_AUX];
 	a.csd = sc_float(w[F_CSD]);
@@ -375,8 +376,10 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	 * ages are whole hours below 2^16 (admission checks it), for which the
 	 * correctly rounded quotient truncates to the integer one (checked for
 	 * every such age), so it is one integer division by a constant instead
-	 * of a float division sequence per agent. */
-	const uint32_t dq = (uint32_t)a.age / 24u;
+	 * of a float division sequence per agent.  The hour counts are binary32
+	 * in the reference; whole numbers below 2^16, they are exact as integers,
+	 * and (float)his is the reference's value wherever it meets a delay. */
+	const uint32_t dq = a.age / 24u;
 	const uint32_t d = dq < 99u ? dq : 99u;
 	float hmr = s == S_EGG ? ip.hmr_egg : s == S_LARVA ? ip.hmr_larva[d] : s == S_PUPA ? ip.hmr_pupa
 	                                                                                 : T.hmr_adult[d];
@@ -384,8 +387,9 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 
 	/* updateAgents.cl:62-70 */
 	const float t = ip.temp;
-	a.age += 1.0f;
-	a.his += 1.0f;
+	a.age += 1u;
+	a.his += 1u;
+	const float his = (float)a.his;
 	uint32_t hbc = a.misc & 0xffffu;
 	/* updateAgents.cl:69-70: csd + cond * (1/((111/(T-16))*14)) in binary64; the
 	 * increment depends on T only (evaluated once per iteration).  T is
@@ -402,56 +406,56 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 	const uint32_t open = ip.open;
 	switch (s) {
 	case S_EGG: /* :72-87 */
-		if (a.his >= a.dly && (open & (1u << S_EGG))) {
-			a.age = 24.0f;
-			a.his = 0.0f;
+		if (his >= a.dly && (open & (1u << S_EGG))) {
+			a.age = 24u;
+			a.his = 0u;
 			a.dly = bm_delay(rnd(&T.ip.seed[4], g), rnd(&T.ip.seed[8], g), (lds_fp)&ip.larva_mean);
 			return S_LARVA;
 		}
 		return S_EGG;
 	case S_LARVA: { /* :89-113 */
-		if (ip.env.larvacideValue > 0.0f && a.his == 1.0f && rnd(&T.ip.seed[8], g) <= ip.env.larvacideValue)
+		if (ip.env.larvacideValue > 0.0f && a.his == 1u && rnd(&T.ip.seed[8], g) <= ip.env.larvacideValue)
 			return S_DEAD;
 		float c = sc_float(a.aux) + ip.dev_rate;
 		a.aux = sc_bits_f(c);
 		if (c >= a.dly && (open & (1u << S_LARVA))) {
-			a.his = 0.0f;
+			a.his = 0u;
 			a.dly = ip.pupa_dly;
 			return S_PUPA;
 		}
 		return S_LARVA;
 	}
 	case S_PUPA: /* :115-129 */
-		if (a.his >= a.dly && (open & (1u << S_PUPA))) {
-			a.age = 0.0f;
-			a.his = 0.0f;
+		if (his >= a.dly && (open & (1u << S_PUPA))) {
+			a.age = 0u;
+			a.his = 0u;
 			a.dly = ip.imm_dly;
 			return S_IMM;
 		}
 		return S_PUPA;
 	case S_IMM: /* :131-143 */
-		if (a.his >= a.dly && (open & (1u << S_IMM))) {
+		if (his >= a.dly && (open & (1u << S_IMM))) {
 			a.dly = ip.ms_dly;
-			a.his = 0.0f;
+			a.his = 0u;
 			return S_MS;
 		}
 		return S_IMM;
 	case S_MS: /* :145-157 */
-		if (a.his >= a.dly && (a.misc >> 31) && (open & (1u << S_MS))) {
-			a.his = 0.0f;
+		if (his >= a.dly && (a.misc >> 31) && (open & (1u << S_MS))) {
+			a.his = 0u;
 			a.dly = ip.bms_dly;
 			return S_BMS;
 		}
 		return S_MS;
 	case S_BMS: /* :159-197 */
 		a.cyc += 1u;
-		if (a.his >= a.dly && (open & (1u << S_BMS)) && rnd(&T.ip.seed[4], g) <= ip.env.bloodmealSuccess) {
+		if (his >= a.dly && (open & (1u << S_BMS)) && rnd(&T.ip.seed[4], g) <= ip.env.bloodmealSuccess) {
 			const float itn = ip.itn_kill;
 			if (itn > 0.0f && rnd(&T.ip.seed[8], g) <= itn) return S_DEAD;
 			const uint32_t human = ip.bite_human < 1.0f ? (rnd(&T.ip.seed[12], g) <= ip.bite_human) : 1u;
 			a.misc = (a.misc & 0xffff0000u) | ((hbc + human) & 0xffffu);
 			ev_add(ev, EV_BITES, 1u);
-			a.his = 0.0f;
+			a.his = 0u;
 			a.dly = ip.bmd_dly;
 			if (a.csd >= 1.0f && human) ev_add(ev, EV_IBITES, 1u);
 			return S_BMD;
@@ -461,13 +465,13 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 		const float irs = ip.irs_kill;
 		if (irs > 0.0f && rnd(&T.ip.seed[8], g) <= irs) return S_DEAD;
 		a.cyc += 1u;
-		if (a.his >= a.dly && (open & (1u << S_BMD))) {
+		if (his >= a.dly && (open & (1u << S_BMD))) {
 			if (a.avail == 0u && (a.misc >> 31)) { /* generateEggs, properties.h:257-266 */
 				uint32_t nb = (a.misc >> 16) & 0x7fffu;
 				a.avail = gen_eggs(rnd(&T.ip.seed[4], g), rnd(&T.ip.seed[8], g), nb, (lds_fp)&ip.batch_mean);
 				a.misc = (a.misc & 0x8000ffffu) | (((nb + 1u) & 0x7fffu) << 16);
 			}
-			a.his = 0.0f;
+			a.his = 0u;
 			a.aux = 0u;
 			return S_GRAVID;
 		}
@@ -493,7 +497,7 @@ __device__ static inline uint32_t agent_step(Ag &a, uint32_t s, uint32_t g, cons
 				ev_add(ev, EV_NCYC, 1u);
 				ev_add(ev, EV_SCYC, a.cyc);
 				a.cyc = 0u;
-				a.his = 0.0f;
+				a.his = 0u;
 				a.dly = ip.bms_dly;
 				return dead ? S_DEAD : S_BMS;
 			}
@@ -518,8 +522,8 @@ __device__ static inline void load_agent(Ag &a, const uint32_t *b, size_t P, uin
 __device__ static inline void newborn(Ag &a, const uint32_t *z, float inc, const float *thr, const float *m,
                                       const float *b, uint32_t g)
 {
-	a.age = 0.0f;
-	a.his = 0.0f;
+	a.age = 0u;
+	a.his = 0u;
 	a.csd = 0.0f;
 	a.aux = 0u;
 	a.avail = 0u;
@@ -1797,7 +1801,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 						pack_agent(x, wv);
 #pragma unroll
 						for (int f = 0; f < NFIELD; ++f) cur[f][o] = wv[f];
-						if (ns2 == S_LARVA) l1de += (uint32_t)x.age / 24u; /* calcL1de.cl: (uint)(age / 24.0f), whole hours */
+						if (ns2 == S_LARVA) l1de += x.age / 24u; /* calcL1de.cl: (uint)(age / 24.0f), whole hours */
 						if (ns2 >= S_IMM && (x.misc >> 31)) {
 							fem += 1u;
 							pinf += x.csd >= 1.0f ? 1u : 0u;
@@ -1981,8 +1985,8 @@ __global__ void k_export(const uint32_t *buf, uint32_t P, const Seg *seg, uint32
 	if (sg.pbase == PB_NEWBORN) newborn(a, nbz, hb.inc, hb.thr, hb.m, hb.b, sg.vbase + (k - sg.k0));
 	else load_agent(a, buf, P, seg_slot(sg.pbase, sg.state, k - sg.k0), sm_fields(sg.state & 0xffu, compact));
 	sampo_agent r;
-	r.ageInHours = a.age;
-	r.hoursInState = a.his;
+	r.ageInHours = (float)a.age;
+	r.hoursInState = (float)a.his;
 	r.delay = a.dly;
 	r.csd = a.csd;
 	r.aux = a.aux;
@@ -2025,8 +2029,8 @@ __global__ void __launch_bounds__(256) k_import(const sampo_agent *in, uint32_t 
 	if (k < n) {
 		const sampo_agent a = in[k];
 		Ag g;
-		g.age = a.ageInHours;
-		g.his = a.hoursInState;
+		g.age = (uint32_t)a.ageInHours; /* whole hours in [0, 2^16): admission checks it */
+		g.his = (uint32_t)a.hoursInState;
 		g.dly = a.delay;
 		g.aux = a.aux;
 		g.csd = a.csd;
