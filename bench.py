@@ -50,8 +50,11 @@ as a sub-object measured in the same process:
   small_run    the 100k-agent single run (latency-bound: its own throughput and
                roofline fraction);
   batched      the 4096-scenario sweep in one engine against the plain
-               one-scenario-at-a-time paths on a sample, with the ratio and
-               bit-identity counts;
+               one-scenario-at-a-time paths on a sample (the engine with its
+               default launch parameters; the engine with the single-run
+               policy: day launches and 512-agent chunks below 1M agents; the
+               reference's driver loop), with the ratios and bit-identity
+               counts;
   serving      a bounded window of the Poisson(50/s) serving trace: p50/p99
                completion latency and throughput.
 """
@@ -180,20 +183,25 @@ def batched_measure(args, workload, log):
     log(f"batched: {bres['agent_steps_per_s']:.3e} agent-steps/s, day {bres['day_ms']:.1f} ms")
     pres, prows = B.run_plain_engine([specs[i] for i in sample], days=1, warm_days=1)
     log(f"plain engine on {len(sample)} scenarios: {pres['agent_steps_per_s']:.3e} agent-steps/s")
+    sres, srows = B.run_plain_engine([specs[i] for i in sample], days=1, warm_days=1, policy="single-run")
+    log(f"plain engine, single-run policy: {sres['agent_steps_per_s']:.3e} agent-steps/s")
     isample = sample[::max(1, len(sample) // max(1, args.icl_sample))][:args.icl_sample]
     ires, irows = B.run_plain_icl([specs[i] for i in isample], days=1)
     log(f"plain icl driver on {len(isample)} scenarios: {ires['agent_steps_per_s']:.3e} agent-steps/s")
     n_cmp = 24 * 2 + 1  # the solo runs' warm-up and timed day
     ident = sum(B.same_rows(brows[i], r, n_cmp) for i, r in zip(sample, prows))
+    ident_s = sum(B.same_rows(brows[i], r, n_cmp) for i, r in zip(sample, srows))
     ident_icl = sum(B.same_rows(brows[i], r, 24) for i, r in zip(isample, irows))
     sub = {
         "workload": desc, "instances": len(specs), "initial_agents": n_total,
         "agent_steps_per_s": bres["agent_steps_per_s"], "agent_days_per_s": bres["agent_steps_per_s"] / 24.0,
         "timed_days": args.batched_days, "warmup_days": args.batched_warmup, "day_ms": bres["day_ms"],
-        "plain_engine": pres, "plain_icl": ires,
+        "plain_engine": pres, "plain_engine_single_run_policy": sres, "plain_icl": ires,
         "speedup_vs_plain_engine": bres["agent_steps_per_s"] / pres["agent_steps_per_s"],
+        "speedup_vs_plain_engine_single_run_policy": bres["agent_steps_per_s"] / sres["agent_steps_per_s"],
         "speedup_vs_plain_icl": bres["agent_steps_per_s"] / ires["agent_steps_per_s"],
-        "identity": {"checked": len(sample), "identical_to_solo": int(ident), "rows_compared": n_cmp,
+        "identity": {"checked": len(sample), "identical_to_solo": int(ident),
+                     "identical_to_solo_single_run_policy": int(ident_s), "rows_compared": n_cmp,
                      "checked_icl": len(isample), "identical_to_icl_driver": int(ident_icl),
                      "rows_compared_icl": 24},
         "roofline": {"bound": "hbm", "achieved": bres["pass_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -204,7 +212,7 @@ def batched_measure(args, workload, log):
         "tuning": {**tinfo, **tun.to_json()},
         "detail": bres,
     }
-    return sub, ident == len(sample) and ident_icl == len(isample)
+    return sub, ident == len(sample) and ident_s == len(sample) and ident_icl == len(isample)
 
 
 def batched_main(args):
@@ -403,8 +411,8 @@ def main():
     ap.add_argument("--tile", default="512x1x6", help="work-group size x agents per thread x min waves per SIMD")
     ap.add_argument("--per-cu", type=int, default=0, help="persistent work-groups of the pass per CU (0: all that fit)")
     ap.add_argument("--chunk", type=int, default=0,
-                    help="single run: minimum agents per chunk (0: 4096, or 512 below 1M initial agents, where "
-                         "4096-agent chunks would leave most work-groups idle)")
+                    help="single run: minimum agents per chunk (0: 3072, or 512 below 1M initial agents, where "
+                         "3072-agent chunks would leave most work-groups idle)")
     ap.add_argument("--target-chunks", type=int, default=0,
                     help="single run: most chunks per iteration (0: two per resident work-group)")
     ap.add_argument("--day", type=int, default=-1,
@@ -434,7 +442,7 @@ def main():
     w = WORKLOADS[args.workload]
     if args.warmup + args.steps > w["days"]:
         sys.exit("warmup + steps exceeds the workload's horizon")
-    chunk = args.chunk or (512 if w["n0"] < 1_000_000 else 4096)
+    chunk = args.chunk or (512 if w["n0"] < 1_000_000 else 3072)
     day = day_groups(args, w)
     res = single_run(w, args, args.workload, args.steps, args.warmup, chunk, day)
     if day:  # the same run by per-iteration launches, for comparison

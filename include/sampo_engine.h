@@ -192,7 +192,7 @@ int sampo_engine_set_day(sampo_engine *e, uint32_t work_groups);
 
 /* Chunking of instances admitted from now on.  A chunk is the run of agents
  * one work-group takes and the granularity of output segments: at least
- * `agents` agents (default 4096), grown so that an instance has at most
+ * `agents` agents (default 3072), grown so that an instance has at most
  * `target_chunks` chunks (0: two per resident work-group of the pass). */
 int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chunks);
 
@@ -201,7 +201,7 @@ int sampo_engine_set_chunk(sampo_engine *e, uint32_t agents, uint32_t target_chu
 int sampo_engine_admit(sampo_engine *e, const sampo_scenario *sc, uint32_t *slot);
 
 /* Device bytes sampo_engine_admit() would allocate for this scenario with
- * the given chunk size (0: the default 4096), for admission by footprint. */
+ * the given chunk size (0: the default 3072), for admission by footprint. */
 uint64_t sampo_engine_footprint(const sampo_scenario *sc, uint32_t chunk);
 
 /* Free a slot (its instance must not be running in an unsynchronised step). */

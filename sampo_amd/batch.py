@@ -220,24 +220,43 @@ def run_batched(specs, tuning, warm_days=1, timed_days=7, phys_factor=1.25, head
             "pass_algorithmic_bytes": pass_bytes, "pass_GBs": pass_bytes / (pass_ms / 1e3) / 1e9}, rows
 
 
-def run_plain_engine(specs, days=1, warm_days=1, keep_rows=True):
-    """Each spec alone in its own engine (default single-run launch
-    parameters); agent-steps/s over the sample and each run's rows."""
+SMALL_RUN = 1_000_000   # below this many initial agents a lone run goes by day launches
+SMALL_CHUNK = 512       # ... with chunks of this many agents
+DAY_WORK_GROUPS = 256
+
+
+def run_plain_engine(specs, days=1, warm_days=1, keep_rows=True, policy="default"):
+    """Each spec alone in its own engine; agent-steps/s over the sample and
+    each run's rows.  policy "default": the engine's default launch
+    parameters (a pass and a prep per iteration, one day graph per day);
+    "single-run": what bench.py uses for a lone run, day launches of 256
+    work-groups and 512-agent chunks below SMALL_RUN initial agents."""
+    import time
     steps = 0
     secs = 0.0
     rows = []
     for sp, kw in W.materialise_all(specs):
         with S.Engine(max_slots=1) as e:
+            day = policy == "single-run" and sp.n0 < SMALL_RUN
+            if day:
+                e.set_chunk(SMALL_CHUNK, 0)
+                e.set_day(DAY_WORK_GROUPS)
             slot = e.admit(**kw)
             for _ in range(warm_days):
                 e.step_days(1)
             e.agent_steps(reset=True)
-            ms = e.time(24 * days, graph=True)
+            if day:  # a day launch per day; the host clock around the timed days
+                e.sync()
+                t0 = time.perf_counter()
+                e.step_days(days)
+                e.sync()
+                secs += time.perf_counter() - t0
+            else:
+                secs += e.time(24 * days, graph=True) / 1e3
             steps += e.agent_steps()
-            secs += ms / 1e3
             if keep_rows:
                 rows.append(e.rows(slot, 0, 24 * (warm_days + days) + 1))
-    return {"instances": len(specs), "agent_steps": steps, "seconds": secs,
+    return {"instances": len(specs), "agent_steps": steps, "seconds": secs, "policy": policy,
             "agent_steps_per_s": steps / secs if secs else 0.0}, rows
 
 
