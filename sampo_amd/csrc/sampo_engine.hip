@@ -1370,8 +1370,8 @@ struct Src {
 /* The pass over a chunk is software-pipelined by tiles of WG * APT agents:
  * while tile t is updated, ranked and written, the loads of tile t + 1 run
  * as direct-to-LDS loads (global_load_lds_dword) into the other of two LDS
- * buffers.  Each lane's slots in a buffer are its own: it loads them, reads
- * them and writes its updated agent back into them. */
+ * buffers.  Each lane's slots in a buffer are its own; the updated agent
+ * stays in the lane's registers until it is stored. */
 /* What a day launch adds to the pass's arguments. */
 struct DayArgs {
 	DaySync *sync;
@@ -1793,6 +1793,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				nt = total <= gridDim.x ? total : gridDim.x + atomicAdd(&ctx->ticket[cls], 1u);
 			}
 			uint32_t ns[APT], lr[APT];
+			uint32_t keep[APT][NFIELD]; /* each survivor's words, in registers until they are stored */
 #pragma unroll 1
 			for (int jx = 0; jx < APT; ++jx) {
 				uint32_t n2 = 0xffu; /* class 2 * input state + moved, 0xff: dead or no agent */
@@ -1803,10 +1804,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t ns2 = agent_step(x, s, me[jx].g, T, ev);
 					if (ns2 < S_DEAD) {
 						n2 = 2u * s + (ns2 != s ? 1u : 0u);
-						uint32_t wv[NFIELD];
-						pack_agent(x, wv);
-#pragma unroll
-						for (int f = 0; f < NFIELD; ++f) cur[f][o] = wv[f];
+						pack_agent(x, keep[jx]);
 						if (ns2 == S_LARVA) l1de += x.age / 24u; /* calcL1de.cl: (uint)(age / 24.0f), whole hours */
 						if (ns2 >= S_IMM && (x.misc >> 31)) {
 							fem += 1u;
@@ -1847,7 +1845,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 					const uint32_t p = s_base[cl] + ((cl & 1u) ? 0u - off : off);
 					uint32_t wv[NFIELD];
 #pragma unroll
-					for (int f = 0; f < NFIELD; ++f) wv[f] = cur[f][o]; /* all read at once: one wait */
+					for (int f = 0; f < NFIELD; ++f) wv[f] = keep[jx][f];
 					auto put = [&](int f) {
 						if constexpr (DAY) /* coherent at agent scope: other work-groups read it next */
 							__hip_atomic_store((uint32_t *)at_slot(fout[f], p), wv[f], __ATOMIC_RELAXED,
