@@ -1404,7 +1404,6 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	__shared__ uint32_t s_acc[NW][3];   /* l1de, females, potentially infective: per wave */
 	__shared__ uint32_t s_ev[NW][NEV]; /* agent_step's event sums of the chunk, per wave */
 	__shared__ uint32_t bufA[NFIELD][TILE], bufB[NFIELD][TILE];
-	__shared__ uint32_t s_sg[TILE], s_ss[TILE]; /* next tile, per lane: virtual index; state | newborn << 8 */
 
 	const uint32_t tid = threadIdx.x, lane = tid & 63u;
 	const uint32_t w = __builtin_amdgcn_readfirstlane(tid >> 6); /* scalar: addresses built on it need no VGPR */
@@ -1467,6 +1466,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 	 * The tile's segments are found by a uniform forward walk; each lane then
 	 * picks its own among those few.  Returns the tile's input-state span. */
 	uint32_t span_lo = 0, span_hi = 0;
+	uint32_t nx_ss[APT], nx_sg[APT]; /* the next tile, this lane's agents: state | newborn << 8; virtual index */
 	auto issue = [&](uint32_t ts, uint32_t te, uint32_t (&buf)[NFIELD][TILE], gu32 *const (&fb)[NFIELD]) {
 		const uint32_t m = s_wm;
 		uint32_t q = next - s_wlo;
@@ -1487,7 +1487,7 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 		for (int jx = 0; jx < APT; ++jx) {
 			const uint32_t k = ts + (uint32_t)jx * WG + tid;
 			const uint32_t o = (uint32_t)jx * WG + tid;
-			s_ss[o] = S_DEAD;
+			nx_ss[jx] = S_DEAD;
 			if (k < te) {
 				uint32_t qq = q_lo;
 				if (!DAY || q_hi - q_lo <= 8u) { /* (the extra test measured 2 % slower in k_pass) */
@@ -1502,8 +1502,8 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 				const uint32_t r = k - s_k0[qq];
 				const uint32_t pb = s_pb[qq], st = s_st[qq];
 				const bool born = pb == PB_NEWBORN;
-				s_ss[o] = (st & 0xffu) | (born ? 0x100u : 0u);
-				s_sg[o] = s_vb[qq] + r;
+				nx_ss[jx] = (st & 0xffu) | (born ? 0x100u : 0u);
+				nx_sg[jx] = s_vb[qq] + r;
 				if (!born) {
 					const uint32_t p = seg_slot(pb, st, r);
 					const uint32_t fm = sm_fields(st & 0xffu, T.compact);
@@ -1758,10 +1758,10 @@ __global__ void __launch_bounds__(WG, WAVES) k_pass(Inst *insts, Ctx *ctx, const
 #pragma unroll
 			for (int jx = 0; jx < APT; ++jx) {
 				const uint32_t o = (uint32_t)jx * WG + tid;
-				const uint32_t ss = s_ss[o];
+				const uint32_t ss = nx_ss[jx];
 				me[jx].s = ss & 0xffu;
 				me[jx].born = ss >> 8;
-				me[jx].g = s_sg[o];
+				me[jx].g = nx_sg[jx];
 				if (me[jx].s < S_DEAD) {
 					if (me[jx].born) {
 						newborn(a[jx], T.ip.nbseed, T.ip.nb_inc, T.ip.hatch_thr, T.ip.hatch_m, T.ip.hatch_b, me[jx].g);
