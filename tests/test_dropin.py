@@ -1,8 +1,12 @@
 """The drop-in recipe, applied to the reference's own driver (CPU).
 
-INTEGRATION.md section 1 lists the edits a maintainer makes so that the
-reference's src/abms.c builds against libsampo_icl.so instead of its
-OpenCL device layer.  This test applies exactly those edits to a scratch
+The recipe is the set of edits a maintainer makes so that the reference's
+src/abms.c builds against libsampo_icl.so instead of its OpenCL device
+layer: include/abms.h includes sampo_icl.h in place of lib_icl.h,
+lib_icl_ext.h and host_types.h, and src/abms.c calls icl_finish and
+icl_wait_for_events where it called clFinish and clWaitForEvents; kernel
+creation stays as it is and the link line names the library.  This test
+applies exactly those edits to a scratch
 copy of the reference's src/ and include/ (nothing of it is kept), compiles
 src/abms.c with the reference Makefile's flags, links it against the
 library and runs the binary's -h path (which touches no device).  It needs
@@ -24,9 +28,8 @@ pytestmark = pytest.mark.skipif(not os.path.isfile(os.path.join(REF, "src", "abm
 
 
 def apply_recipe(tree):
-    """The edits of INTEGRATION.md section 1, steps 1 and 2 (steps 3 and 4
-    change nothing in the source: kernel creation stays, the link line is
-    the command below)."""
+    """The recipe's two source edits (its other steps change nothing in the
+    source: kernel creation stays, the link line is the command below)."""
     p = os.path.join(tree, "include", "abms.h")
     s = open(p).read()
     old = '#include "lib_icl.h"\n#include "lib_icl_ext.h"\n\n#include "host_types.h"\n'
